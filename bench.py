@@ -1,0 +1,240 @@
+#!/usr/bin/env python3
+"""Throughput of the operator-split monodomain step on one MI355X.
+
+Workload (the metric's configuration): a 400 x 400 x 100 = 16,000,000-cell
+cuboid at 100 um, O'Hara-Rudy 2011 endocardial cells (41 states), anisotropic
+conductivity sigma_l / sigma_t = 0.24 / 0.045 S/m with fibres in the xy plane
+rotating linearly with depth from -60 to +60 degrees (19-point rows),
+planar stimulus -50 uA/uF on x < 500 um for 2 ms from t = 0, dt = 0.02 ms,
+implicit Euler diffusion solved by CG to a relative residual of 1e-10.
+It runs twice: FP64 plain, then FP32 states + FP32 CG inside FP64 defect
+correction.  One step = ionic step + RHS update + converged diffusion solve for
+every cell; the metric is cell-timesteps per second.
+
+Inputs are synthetic (generated here) and stay resident in HBM during the
+timed region.  `value` is the FP32+refine rate; the FP64 rate, the speed-up
+and the roofline of the dominant kernel are in the same JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-timesteps/s at 16M cells: FP64 plain vs FP32+refine at <=0.5ms LAT err"
+
+
+def workload_desc(shape, precision):
+    import monoalg3d_c_amd as m
+    nz = shape[2]
+    t6 = m.layered_fibre_tensors(nz, -60.0, 60.0, 0.00024, 0.000045)
+    return dict(model=m.MODEL_ORD2011, precision=precision, aniso=True, tensor=t6,
+                tensor_source=m.TENSOR_PER_LAYER, cg_rel_tol=1e-10, cg_tol=0.0, cg_max_its=500,
+                stim=(0.0, 2.0, -50.0, 500.0, 0.0))
+
+
+def bytes_per_cell_step(neq, w_state, k_iter, mixed, outer):
+    """Algorithmic HBM bytes per cell-step (compulsory traffic, perfect stencil reuse).
+
+    ionic: read neq-1 states + V (FP64), write neq-1 states + b (FP64)
+    FP64 CG: residual 4 words (x, b in; r, p out) + 11 words per iteration
+    (q = A p: p in, q out; x += a p: x, p in, x out; r -= a q: r, q in, r out;
+    p = r + b p: r, p in, p out).  Class-table operator: 0 coefficient bytes.
+    FP32+refine: per outer sweep residual (x, b FP64 in; r, p, d FP32 out) and
+    correction (d FP32 in, x FP64 in/out); inner iterations 11 FP32 words.
+    """
+    ionic = 2 * (neq - 1) * w_state + 8 + 8
+    if not mixed:
+        return ionic + 4 * 8 + k_iter * 11 * 8
+    return ionic + outer * (8 + 8 + 3 * 4) + max(outer - 1, 0) * (4 + 8 + 8) + k_iter * 11 * 4
+
+
+def run_precision(shape, precision, steps, warmup):
+    import monoalg3d_c_amd as m
+    n = shape[0] * shape[1] * shape[2]
+    t0 = time.time()
+    tis = m.Tissue(shape, **workload_desc(shape, precision))
+    setup_s = time.time() - t0
+    tis.step(warmup)
+    tis.sync()
+    s0 = tis.stats
+    tis.set_timing(True)
+    t0 = time.perf_counter()
+    tis.step(steps)
+    tis.sync()
+    wall = time.perf_counter() - t0
+    s1 = tis.stats
+    tis.close()
+    its = (s1["cg_iterations_total"] - s0["cg_iterations_total"]) / steps
+    inner = (s1["inner_iterations_total"] - s0["inner_iterations_total"]) / steps
+    outer = (s1["outer_sweeps_total"] - s0["outer_sweeps_total"]) / steps
+    ionic_ms = (s1["ionic_ms_total"] - s0["ionic_ms_total"]) / steps
+    solve_ms = (s1["solve_ms_total"] - s0["solve_ms_total"]) / steps
+    return dict(rate=n * steps / wall, ms_per_step=1e3 * wall / steps, cg_its=its, inner_its=inner, outer=outer,
+                ionic_ms=ionic_ms, solve_ms=solve_ms, setup_s=setup_s, class_operator=s1["class_operator"],
+                stencil_points=s1["stencil_points"])
+
+
+def cpu_baseline(max_threads=16):
+    """Reference CPU path (oracle/_ref MonoAlg3D, OpenMP) on a bounded sample:
+    a 128 x 128 x 32 cuboid with ORd 2011 and anisotropic 0.24/0.045 S/m
+    conductivity (uniform in-plane fibre at 45 degrees: same 19-point rows),
+    30 steps; rate = cells * steps / Resolution Time."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "bin", "MonoAlg3D")
+    if not os.path.exists(exe):
+        return None
+    threads = max(1, min(os.cpu_count() or 1, max_threads))
+    nx, ny, nz, steps = 128, 128, 32, 30
+    out_dir = os.path.join(ROOT, "gpurun_out", "cpu_baseline")
+    os.makedirs(out_dir, exist_ok=True)
+    f = np.cos(np.pi / 4)
+    ini = f"""[main]
+num_threads = {threads}
+dt_pde = 0.02
+simulation_time = {0.02 * (steps - 1):.6f}
+abort_on_no_activity = false
+use_adaptivity = false
+[update_monodomain]
+main_function=update_monodomain_default
+[assembly_matrix]
+init_function=set_initial_conditions_fvm
+sigma_l=0.00024
+sigma_t=0.000045
+sigma_n=0.000045
+f=[{f},{f},0]
+main_function=anisotropic_sigma_assembly_matrix
+[linear_system_solver]
+tolerance=1e-16
+use_preconditioner=no
+max_iterations=500
+main_function=conjugate_gradient
+[domain]
+name=Cuboid Mesh
+start_dx=100.0
+start_dy=100.0
+start_dz=100.0
+side_length_x={nx * 100.0}
+side_length_y={ny * 100.0}
+side_length_z={nz * 100.0}
+main_function=initialize_grid_with_cuboid_mesh
+[ode_solver]
+dt=0.02
+use_gpu=no
+library_file=shared_libs/libohara_rudy_endo_2011.so
+[stim_plain]
+start = 0.0
+duration = 2.0
+current = -50.0
+x_limit = 500.0
+main_function=stim_if_x_less_than
+"""
+    path = os.path.join(out_dir, "cpu_baseline.ini")
+    with open(path, "w") as fh:
+        fh.write(ini)
+    r = subprocess.run([exe, "-c", path], cwd=os.path.join(ROOT, "oracle", "_ref"), capture_output=True, text=True,
+                       timeout=600)
+    if r.returncode != 0:
+        return None
+    res_us, loops = None, 0
+    for line in r.stdout.splitlines():
+        if line.startswith("Resolution Time:"):
+            res_us = float(line.split(":")[1].split("μs")[0])
+        if line.startswith("t = "):
+            loops += 1
+    if not res_us:
+        return None
+    iters = int(round(0.02 * (steps - 1) / 0.02)) + 1
+    n = nx * ny * nz
+    return dict(value=n * iters / (res_us * 1e-6), unit="cell-timesteps/s", cores=threads, kind="reference",
+                sample=f"reference MonoAlg3D CPU (OpenMP {threads} threads), {nx}x{ny}x{nz} cuboid, ORd 2011, "
+                       f"anisotropic 19-point, {iters} steps from t=0 incl. stimulus; rate = cells*steps/Resolution Time")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--shape", type=str, default="400,400,100")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "fp32", "both"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is a design "
+              f"note only, see DESIGN.md)", file=sys.stderr)
+        return 2
+    import monoalg3d_c_amd as m
+    if m.device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        return 1
+    shape = tuple(int(s) for s in args.shape.split(","))
+    n = shape[0] * shape[1] * shape[2]
+    neq = m.model_neq(m.MODEL_ORD2011)
+    res = {}
+    if args.only in ("fp64", "both"):
+        res["fp64"] = run_precision(shape, m.FP64, args.steps, args.warmup)
+        print(f"[bench] fp64 {res['fp64']}", file=sys.stderr, flush=True)
+    if args.only in ("fp32", "both"):
+        res["fp32"] = run_precision(shape, m.FP32_REFINE, args.steps, args.warmup)
+        print(f"[bench] fp32+refine {res['fp32']}", file=sys.stderr, flush=True)
+    head = res.get("fp32") or res["fp64"]
+    mixed = "fp32" in res
+
+    # roofline of the dominant phase (by device time)
+    rf = {}
+    for key, r in res.items():
+        w = 8 if key == "fp64" else 4
+        ionic_bytes = (2 * (neq - 1) * w + 16) * n
+        solve_bytes = (bytes_per_cell_step(neq, w, r["cg_its"] if key == "fp64" else r["inner_its"], key == "fp32",
+                                           r["outer"]) - (2 * (neq - 1) * w + 16)) * n
+        rf[key] = dict(
+            ionic=dict(achieved=ionic_bytes / (r["ionic_ms"] * 1e-3) / 1e9, bytes=ionic_bytes, ms=r["ionic_ms"]),
+            solve=dict(achieved=solve_bytes / (r["solve_ms"] * 1e-3) / 1e9, bytes=solve_bytes, ms=r["solve_ms"]),
+            step_bytes_per_cell=bytes_per_cell_step(neq, w, r["cg_its"] if key == "fp64" else r["inner_its"],
+                                                    key == "fp32", r["outer"]))
+    hk = "fp32" if mixed else "fp64"
+    dom = "ionic" if res[hk]["ionic_ms"] >= res[hk]["solve_ms"] else "solve"
+    d = rf[hk][dom]
+    roofline = {"bound": "hbm", "achieved": round(d["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(d["achieved"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": f"{dom} phase ({hk})"}
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    out = {
+        "metric": METRIC,
+        "value": head["rate"],
+        "unit": "cell-timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64refine" if mixed else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"cuboid {shape[0]}x{shape[1]}x{shape[2]} cells at 100 um, ORd 2011 endo, "
+                               "anisotropic 0.24/0.045 S/m fibres -60..+60 deg over depth, planar stimulus, "
+                               "CG rel. residual 1e-10, dt 0.02 ms",
+                   "cells": n, "precision_value": "fp32+refine" if mixed else "fp64"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "fp64": res.get("fp64"),
+        "fp32_refine": res.get("fp32"),
+        "speedup_fp32_refine_vs_fp64": (res["fp32"]["rate"] / res["fp64"]["rate"]) if len(res) == 2 else None,
+        "phase_roofline": rf,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

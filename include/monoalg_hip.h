@@ -1,0 +1,167 @@
+/*
+ * monoalg_hip.h - C ABI of the MI355X monodomain timestep library
+ * (libmonoalg_hip.so).  Plain pointers and sizes only.
+ *
+ * The library is the device side of MonoAlg3D's operator-split monodomain
+ * step.  Each entry point names the reference interface it replaces:
+ *
+ *   mono_tissue_create      solve_monodomain set-up (monodomain_solver.c:39-614):
+ *                           SET_ODE_INITIAL_CONDITIONS_GPU (ode_solver.h:46,
+ *                           ode_solver.c:235-286), INIT_ASSEMBLY_MATRIX
+ *                           set_initial_conditions_fvm (matrix_assembly.c:27-44),
+ *                           ASSEMBLY_MATRIX (assembly_matrix_config.h:15;
+ *                           matrix_assembly.c:334-514), SET_SPATIAL_STIM
+ *                           stim_if_x_less_than (stimuli.c:56-91) and
+ *                           INIT_LINEAR_SYSTEM (linear_system_solver_config.h:18).
+ *   mono_tissue_step        the body of the time loop (monodomain_solver.c:616-925):
+ *                           SOLVE_MODEL_ODES (ode_solver.h:37), UPDATE_MONODOMAIN
+ *                           (update_monodomain_config.h:15-19) and
+ *                           SOLVE_LINEAR_SYSTEM (linear_system_solver_config.h:12-16),
+ *                           all device-resident.
+ *   mono_tissue_get_v /     cell->v read-back as done by the save plugins
+ *   mono_tissue_get_state   (save_mesh.c:248-420) and the SV dump of
+ *                           save_state.c:16-155.
+ *   mono_tissue_destroy     END_LINEAR_SYSTEM + the SV free of ode_solver.c:149-158.
+ *   mono_ionic_run          SET_ODE_INITIAL_CONDITIONS_GPU + SOLVE_MODEL_ODES on
+ *                           caller-given states and per-cell stimuli
+ *                           (ode_solver.h:37-47; the GPU plugins'
+ *                           solve_model_odes_gpu take a host stim array).
+ *   mono_assemble           ASSEMBLY_MATRIX rows (matrix_assembly.c:334-514) as a
+ *                           dense 27-point stencil per cell.
+ *   mono_cg_stencil /       SOLVE_LINEAR_SYSTEM conjugate_gradient
+ *   mono_cg_csr             (linear_system_solver.c:185-200; CSR as built by
+ *                           grid_to_csr, grid.c:573-649).
+ *
+ * Errors: every int-returning function returns 0 on success and a negative
+ * value on failure; mono_last_error() then describes it (the reference's
+ * plugins exit() instead, e.g. log_error_and_exit).
+ *
+ * Cell order is lexicographic on the structured box: cell = x + nx*(y + ny*z),
+ * cell centre = ((x+0.5)dx, (y+0.5)dy, (z+0.5)dz).
+ */
+#ifndef MONOALG_HIP_H
+#define MONOALG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ionic models (library_file of [ode_solver]) */
+enum {
+    MONO_MODEL_TT2006 = 0,  /* libten_tusscher_2006.so, 19 states */
+    MONO_MODEL_ORD2011 = 1, /* libohara_rudy_endo_2011.so, 41 states */
+    MONO_MODEL_MS2003 = 2   /* libmitchell_shaeffer_2003.so, 2 states */
+};
+
+/* arithmetic of the ionic step and of the diffusion solve */
+enum {
+    MONO_FP64 = 0,        /* FP64 states, FP64 CG (the reference's CELL_MODEL_REAL_DOUBLE) */
+    MONO_FP32_REFINE = 1  /* FP32 states, FP32 CG inside FP64 defect correction */
+};
+
+/* how the conductivity tensor is given (anisotropic_sigma_assembly_matrix) */
+enum {
+    MONO_TENSOR_UNIFORM = 0,   /* 6 doubles */
+    MONO_TENSOR_PER_LAYER = 1, /* nz x 6 doubles (fibres varying with depth) */
+    MONO_TENSOR_PER_CELL = 2   /* n x 6 doubles */
+};
+
+typedef struct mono_tissue_desc {
+    /* structured box of cells (initialize_grid_with_cuboid_mesh) */
+    int nx, ny, nz;
+    double dx, dy, dz;              /* um */
+    int model;                      /* MONO_MODEL_* */
+    int precision;                  /* MONO_FP64 | MONO_FP32_REFINE */
+    double dt_pde, dt_ode;          /* ms; ODE sub-steps = (int)(dt_pde/dt_ode) */
+    double beta, cm;                /* 1/um, uF/cm^2 ([main] defaults 0.14, 1.0) */
+    /* operator: aniso = 0 -> homogeneous_sigma_assembly_matrix with the diagonal
+     * of the tensor as (sigma_x, sigma_y, sigma_z); aniso = 1 ->
+     * anisotropic_sigma_assembly_matrix.  Tensor components are
+     * (sxx, syy, szz, sxy, sxz, syz) in mS/um. */
+    int aniso;
+    int tensor_source;              /* MONO_TENSOR_* */
+    const double *tensor;           /* host */
+    const double *g_cell;           /* optional host per-cell conductivity factor */
+    const uint8_t *active;          /* optional host per-cell 0/1 (inactive cells) */
+    /* linear solver ([linear_system_solver]) */
+    double cg_tol;                  /* absolute r.r threshold (reference `tolerance`) */
+    double cg_rel_tol;              /* > 0: stop on ||r|| <= rel * ||b|| instead */
+    int cg_max_its;                 /* `max_iterations` */
+    int cg_jacobi;                  /* `use_preconditioner` */
+    double inner_rel;               /* FP32_REFINE: inner relative tolerance (0 = 1e-4) */
+    /* stimulus ([stim_*] stim_if_x_less_than) */
+    double stim_start, stim_duration, stim_current, stim_x_limit, stim_period;
+} mono_tissue_desc;
+
+typedef struct mono_stats {
+    int64_t steps;                  /* loop iterations done */
+    double time;                    /* current simulation time (ms) */
+    int64_t cg_iterations_total;    /* sum of per-step iteration counts */
+    int last_cg_iterations;         /* reference count (1 + CG iterations) */
+    double last_cg_error;           /* last r.r */
+    int last_outer;                 /* FP32_REFINE: outer sweeps of the last step */
+    int64_t inner_iterations_total; /* CG iterations actually run (all sweeps) */
+    int64_t outer_sweeps_total;
+    int nan_detected;               /* the reference stops on a NaN solver error */
+    int class_operator;             /* 1: operator stored as per-layer class table */
+    int stencil_points;             /* stored non-zeros per row */
+    double ionic_ms_total;          /* device time in the ionic kernel (timing on) */
+    double solve_ms_total;          /* device time in the diffusion solve (timing on) */
+} mono_stats;
+
+typedef struct mono_tissue mono_tissue;
+
+const char *mono_last_error(void);
+int mono_device_count(void);
+
+int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out);
+int mono_tissue_destroy(mono_tissue *t);
+/* advance nsteps loop iterations (ODE + update + solve); returns 1 if a NaN
+ * solver error stopped the run (monodomain_solver.c:746-754) */
+int mono_tissue_step(mono_tissue *t, int nsteps);
+int mono_tissue_sync(mono_tissue *t);
+int64_t mono_tissue_num_cells(const mono_tissue *t);
+int mono_tissue_neq(const mono_tissue *t);
+int mono_tissue_get_v(mono_tissue *t, double *host_v);               /* n doubles */
+int mono_tissue_set_v(mono_tissue *t, const double *host_v);
+int mono_tissue_get_state(mono_tissue *t, double *host_sv);          /* neq x n, row-major by state */
+int mono_tissue_set_state(mono_tissue *t, const double *host_sv);
+int mono_tissue_get_stats(const mono_tissue *t, mono_stats *out);
+/* 1 = time the ionic and solve phases with HIP events (adds syncs) */
+int mono_tissue_set_timing(mono_tissue *t, int on);
+/* device pointers for zero-copy consumers: v (FP64, n), sv (neq rows of
+ * `pitch` elements, FP64 or FP32 by precision), b (FP64, n) */
+int mono_tissue_device_ptrs(mono_tissue *t, void **v, void **sv, size_t *pitch, void **b);
+
+/* Standalone ionic solve: n cells, states AoS (n x neq, FP64, in/out), per-cell
+ * stimulus (n doubles, may be NULL), nsteps ODE steps of dt.  precision selects
+ * the FP64 or FP32 kernel. */
+int mono_ionic_run(int model, int precision, double *states, int64_t n, const double *stim, double dt, int nsteps);
+int mono_model_neq(int model);
+/* default initial state (neq doubles) and INITIAL_V of a model */
+int mono_model_initial_state(int model, double *out);
+double mono_model_initial_v(int model);
+
+/* Assemble the operator of a desc into coef27 (27 x n doubles, slot-major,
+ * slot = (oz+1)*9 + (oy+1)*3 + (ox+1), slot 13 = diagonal); alpha_out gets
+ * ALPHA(beta, cm, dt_pde, dx, dy, dz). */
+int mono_assemble(const mono_tissue_desc *desc, double *coef27, double *alpha_out);
+
+/* CG on a 27-slot stencil operator (host arrays, slot-major as above):
+ * solves A x = b from the warm start x.  precision: MONO_FP64 or
+ * MONO_FP32_REFINE.  iterations/error follow the reference's counting. */
+int mono_cg_stencil(int nx, int ny, int nz, const double *coef27, const uint8_t *active, const double *b,
+                    double *x, double tol, double rel_tol, int max_its, int jacobi, int precision,
+                    int *iterations, double *error);
+/* CG on a CSR matrix (host arrays), FP64. */
+int mono_cg_csr(int n, const int64_t *row_ptr, const int32_t *col, const double *val, const double *b,
+                double *x, double tol, int max_its, int jacobi, int *iterations, double *error);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

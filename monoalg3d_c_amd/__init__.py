@@ -1,0 +1,226 @@
+"""MI355X-native operator-split monodomain timestep (MonoAlg3D drop-in).
+
+Host-side mirror of the reference's plugin interface for this path:
+  - Tissue: the solve_monodomain loop body (ionic step + RHS update + diffusion
+    solve) on one structured box, device-resident;
+  - ionic_run / assemble / cg_stencil / cg_csr: the SOLVE_MODEL_ODES,
+    ASSEMBLY_MATRIX and SOLVE_LINEAR_SYSTEM plugins on their own;
+  - workloads: the cuboid, layered-fibre and stimulus set-ups of the benchmark
+    configurations.
+All compute runs in libmonoalg_hip.so (HIP, gfx950).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (FP32_REFINE, FP64, MODEL_MS2003, MODEL_ORD2011, MODEL_TT2006, TENSOR_PER_CELL,  # noqa: F401
+                   TENSOR_PER_LAYER, TENSOR_UNIFORM, MonoError, Stats, TissueDesc, check, lib, ptr)
+
+MODELS = {"tt2006": MODEL_TT2006, "ord2011": MODEL_ORD2011, "ms2003": MODEL_MS2003}
+
+
+def model_neq(model: int) -> int:
+    return lib().mono_model_neq(model)
+
+
+def model_initial_state(model: int) -> np.ndarray:
+    out = np.zeros(model_neq(model), np.float64)
+    check(lib().mono_model_initial_state(model, ptr(out)), "mono_model_initial_state")
+    return out
+
+
+def model_initial_v(model: int) -> float:
+    return lib().mono_model_initial_v(model)
+
+
+def alpha(beta=0.14, cm=1.0, dt=0.02, dx=100.0, dy=100.0, dz=100.0) -> float:
+    """ALPHA of monodomain/constants.h:10-11 (float-rounded 1e-8 factor)."""
+    return float(((beta * cm) / dt) * float(np.float32(1e-8)) * ((dx * dy) * dz))
+
+
+def iso_tensor(sx: float, sy: float | None = None, sz: float | None = None) -> np.ndarray:
+    sy = sx if sy is None else sy
+    sz = sx if sz is None else sz
+    return np.array([sx, sy, sz, 0.0, 0.0, 0.0], np.float64)
+
+
+def normalise(f) -> np.ndarray:
+    """Unit fibre as the reference's fibre reader makes it (assembly_common.c:1141-1146)."""
+    f = np.asarray(f, np.float64)
+    m = np.sqrt((f[0] * f[0]) + (f[1] * f[1]) + (f[2] * f[2]))
+    return np.array([f[0] / m, f[1] / m, f[2] / m])
+
+
+def fibre_tensor(f, sigma_l: float, sigma_t: float) -> np.ndarray:
+    """calc_tensor2 (assembly_common.c:1121-1139): D = (sl - st) f f^T + st I."""
+    f = np.asarray(f, np.float64)
+    D = np.empty((3, 3))
+    for i in range(3):
+        for j in range(3):
+            D[i, j] = (f[i] * f[j]) * (sigma_l - sigma_t) + (sigma_t if i == j else 0.0)
+    return np.array([D[0, 0], D[1, 1], D[2, 2], D[0, 1], D[0, 2], D[1, 2]])
+
+
+def layered_fibre_tensors(nz: int, theta0_deg: float, theta1_deg: float, sigma_l: float, sigma_t: float) -> np.ndarray:
+    """Per-layer tensors for fibres in the xy plane rotating linearly with depth:
+    theta_k = theta0 + (theta1 - theta0) k / (nz - 1), f = (cos, sin, 0) normalised."""
+    out = np.zeros((nz, 6))
+    for k in range(nz):
+        th = np.deg2rad(theta0_deg + (theta1_deg - theta0_deg) * (k / (nz - 1) if nz > 1 else 0.0))
+        out[k] = fibre_tensor(normalise([np.cos(th), np.sin(th), 0.0]), sigma_l, sigma_t)
+    return out
+
+
+def make_desc(shape, h=(100.0, 100.0, 100.0), model=MODEL_TT2006, precision=FP64, dt=0.02, dt_ode=None, beta=0.14,
+              cm=1.0, aniso=False, tensor=None, tensor_source=TENSOR_UNIFORM, g_cell=None, active=None,
+              cg_tol=1e-16, cg_rel_tol=0.0, cg_max_its=200, jacobi=False, inner_rel=1e-4,
+              stim=(0.0, 2.0, -50.0, 500.0, 0.0)):
+    """Build a TissueDesc; returns (desc, keepalive) where keepalive holds the arrays."""
+    d = TissueDesc()
+    d.nx, d.ny, d.nz = shape
+    d.dx, d.dy, d.dz = h
+    d.model, d.precision = model, precision
+    d.dt_pde, d.dt_ode = dt, (dt if dt_ode is None else dt_ode)
+    d.beta, d.cm = beta, cm
+    d.aniso = int(aniso)
+    if tensor is None:
+        tensor = iso_tensor(0.00013)
+    tensor = np.ascontiguousarray(tensor, np.float64)
+    d.tensor_source = tensor_source
+    d.tensor = tensor.ctypes.data
+    keep = [tensor]
+    if g_cell is not None:
+        g_cell = np.ascontiguousarray(g_cell, np.float64)
+        keep.append(g_cell)
+        d.g_cell = g_cell.ctypes.data
+    if active is not None:
+        active = np.ascontiguousarray(active, np.uint8)
+        keep.append(active)
+        d.active = active.ctypes.data
+    d.cg_tol, d.cg_rel_tol, d.cg_max_its, d.cg_jacobi = cg_tol, cg_rel_tol, cg_max_its, int(jacobi)
+    d.inner_rel = inner_rel
+    d.stim_start, d.stim_duration, d.stim_current, d.stim_x_limit, d.stim_period = stim
+    return d, keep
+
+
+class Tissue:
+    """Device-resident monodomain run on a structured box (see make_desc for arguments)."""
+
+    def __init__(self, shape, **kw):
+        self.desc, self._keep = make_desc(shape, **kw)
+        self._h = C.c_void_p()
+        check(lib().mono_tissue_create(C.byref(self.desc), C.byref(self._h)), "mono_tissue_create")
+        self.n = int(lib().mono_tissue_num_cells(self._h))
+        self.neq = int(lib().mono_tissue_neq(self._h))
+        self.shape = tuple(shape)
+
+    def close(self) -> None:
+        if self._h:
+            check(lib().mono_tissue_destroy(self._h), "mono_tissue_destroy")
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, k: int = 1) -> int:
+        """Advance k loop iterations; returns 1 if a NaN stopped the run."""
+        return check(lib().mono_tissue_step(self._h, k), "mono_tissue_step")
+
+    def sync(self) -> None:
+        check(lib().mono_tissue_sync(self._h), "mono_tissue_sync")
+
+    def set_timing(self, on: bool) -> None:
+        check(lib().mono_tissue_set_timing(self._h, int(on)), "mono_tissue_set_timing")
+
+    @property
+    def v(self) -> np.ndarray:
+        out = np.empty(self.n, np.float64)
+        check(lib().mono_tissue_get_v(self._h, ptr(out)), "mono_tissue_get_v")
+        return out
+
+    @v.setter
+    def v(self, val) -> None:
+        a = np.ascontiguousarray(val, np.float64)
+        assert a.size == self.n
+        check(lib().mono_tissue_set_v(self._h, ptr(a)), "mono_tissue_set_v")
+
+    @property
+    def state(self) -> np.ndarray:
+        """(neq, n) float64; row 0 is not maintained on the fast path (V lives in .v)."""
+        out = np.empty((self.neq, self.n), np.float64)
+        check(lib().mono_tissue_get_state(self._h, ptr(out)), "mono_tissue_get_state")
+        return out
+
+    @state.setter
+    def state(self, val) -> None:
+        a = np.ascontiguousarray(val, np.float64)
+        assert a.shape == (self.neq, self.n)
+        check(lib().mono_tissue_set_state(self._h, ptr(a)), "mono_tissue_set_state")
+
+    @property
+    def stats(self) -> dict:
+        s = Stats()
+        check(lib().mono_tissue_get_stats(self._h, C.byref(s)), "mono_tissue_get_stats")
+        return s.as_dict()
+
+    def device_ptrs(self):
+        v, sv, b = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        pitch = C.c_size_t()
+        check(lib().mono_tissue_device_ptrs(self._h, C.byref(v), C.byref(sv), C.byref(pitch), C.byref(b)),
+              "mono_tissue_device_ptrs")
+        return v.value, sv.value, pitch.value, b.value
+
+
+def ionic_run(model: int, states: np.ndarray, stim: np.ndarray | None, dt: float, nsteps: int = 1,
+              precision: int = FP64) -> np.ndarray:
+    """SOLVE_MODEL_ODES on AoS states (n, neq); returns the new states."""
+    s = np.array(states, np.float64, copy=True, order="C")
+    n = s.shape[0]
+    st = None if stim is None else np.ascontiguousarray(stim, np.float64)
+    check(lib().mono_ionic_run(model, precision, ptr(s), n, ptr(st), dt, nsteps), "mono_ionic_run")
+    return s
+
+
+def assemble(shape, **kw):
+    """ASSEMBLY_MATRIX rows as (n, 27) FP64 stencils; returns (coef, alpha)."""
+    d, keep = make_desc(shape, **kw)
+    n = shape[0] * shape[1] * shape[2]
+    out = np.zeros(27 * n, np.float64)
+    a = C.c_double()
+    check(lib().mono_assemble(C.byref(d), ptr(out), C.byref(a)), "mono_assemble")
+    return out.reshape(27, n).T.copy(), a.value
+
+
+def cg_stencil(shape, coef27, b, x0, tol=1e-16, rel_tol=0.0, max_its=200, jacobi=False, precision=FP64, active=None):
+    """CG on an (n, 27) stencil operator; returns (x, iterations, error)."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    c = np.ascontiguousarray(np.asarray(coef27, np.float64).reshape(n, 27).T)
+    bb = np.ascontiguousarray(b, np.float64)
+    x = np.array(x0, np.float64, copy=True)
+    act = None if active is None else np.ascontiguousarray(active, np.uint8)
+    its, err = C.c_int(), C.c_double()
+    check(lib().mono_cg_stencil(nx, ny, nz, ptr(c), ptr(act), ptr(bb), ptr(x), tol, rel_tol, max_its, int(jacobi),
+                                precision, C.byref(its), C.byref(err)), "mono_cg_stencil")
+    return x, its.value, err.value
+
+
+def cg_csr(row_ptr, col, val, b, x0, tol=1e-16, max_its=200, jacobi=False):
+    rp = np.ascontiguousarray(row_ptr, np.int64)
+    cc = np.ascontiguousarray(col, np.int32)
+    vv = np.ascontiguousarray(val, np.float64)
+    bb = np.ascontiguousarray(b, np.float64)
+    x = np.array(x0, np.float64, copy=True)
+    its, err = C.c_int(), C.c_double()
+    check(lib().mono_cg_csr(len(rp) - 1, ptr(rp), ptr(cc), ptr(vv), ptr(bb), ptr(x), tol, max_its, int(jacobi),
+                            C.byref(its), C.byref(err)), "mono_cg_csr")
+    return x, its.value, err.value
+
+
+def device_count() -> int:
+    return lib().mono_device_count()

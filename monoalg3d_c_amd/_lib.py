@@ -1,0 +1,113 @@
+"""ctypes binding of libmonoalg_hip.so (the C ABI declared in include/monoalg_hip.h).
+
+The compute path is the HIP library; if it is missing this module raises
+instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libmonoalg_hip.so")
+
+MODEL_TT2006, MODEL_ORD2011, MODEL_MS2003 = 0, 1, 2
+FP64, FP32_REFINE = 0, 1
+TENSOR_UNIFORM, TENSOR_PER_LAYER, TENSOR_PER_CELL = 0, 1, 2
+
+# the symbols include/monoalg_hip.h declares
+EXPORTED = (
+    "mono_last_error", "mono_device_count", "mono_tissue_create", "mono_tissue_destroy", "mono_tissue_step",
+    "mono_tissue_sync", "mono_tissue_num_cells", "mono_tissue_neq", "mono_tissue_get_v", "mono_tissue_set_v",
+    "mono_tissue_get_state", "mono_tissue_set_state", "mono_tissue_get_stats", "mono_tissue_set_timing",
+    "mono_tissue_device_ptrs", "mono_ionic_run", "mono_model_neq", "mono_model_initial_state",
+    "mono_model_initial_v", "mono_assemble", "mono_cg_stencil", "mono_cg_csr",
+)
+
+
+class MonoError(RuntimeError):
+    pass
+
+
+class TissueDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("model", C.c_int), ("precision", C.c_int),
+        ("dt_pde", C.c_double), ("dt_ode", C.c_double),
+        ("beta", C.c_double), ("cm", C.c_double),
+        ("aniso", C.c_int), ("tensor_source", C.c_int),
+        ("tensor", C.c_void_p), ("g_cell", C.c_void_p), ("active", C.c_void_p),
+        ("cg_tol", C.c_double), ("cg_rel_tol", C.c_double), ("cg_max_its", C.c_int), ("cg_jacobi", C.c_int),
+        ("inner_rel", C.c_double),
+        ("stim_start", C.c_double), ("stim_duration", C.c_double), ("stim_current", C.c_double),
+        ("stim_x_limit", C.c_double), ("stim_period", C.c_double),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64), ("time", C.c_double), ("cg_iterations_total", C.c_int64),
+        ("last_cg_iterations", C.c_int), ("last_cg_error", C.c_double), ("last_outer", C.c_int),
+        ("inner_iterations_total", C.c_int64), ("outer_sweeps_total", C.c_int64), ("nan_detected", C.c_int),
+        ("class_operator", C.c_int), ("stencil_points", C.c_int), ("ionic_ms_total", C.c_double),
+        ("solve_ms_total", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library (raises MonoError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MonoError(f"{LIB_PATH} not found: build it with `make -C monoalg3d_c_amd/csrc` "
+                        "(or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, i64 = C.c_void_p, C.POINTER(C.c_double), C.c_int64
+    L.mono_last_error.restype = C.c_char_p
+    L.mono_device_count.restype = C.c_int
+    L.mono_tissue_create.argtypes = [C.POINTER(TissueDesc), C.POINTER(vp)]
+    L.mono_tissue_destroy.argtypes = [vp]
+    L.mono_tissue_step.argtypes = [vp, C.c_int]
+    L.mono_tissue_sync.argtypes = [vp]
+    L.mono_tissue_num_cells.argtypes = [vp]
+    L.mono_tissue_num_cells.restype = i64
+    L.mono_tissue_neq.argtypes = [vp]
+    L.mono_tissue_get_v.argtypes = [vp, vp]
+    L.mono_tissue_set_v.argtypes = [vp, vp]
+    L.mono_tissue_get_state.argtypes = [vp, vp]
+    L.mono_tissue_set_state.argtypes = [vp, vp]
+    L.mono_tissue_get_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.mono_tissue_set_timing.argtypes = [vp, C.c_int]
+    L.mono_tissue_device_ptrs.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_size_t), C.POINTER(vp)]
+    L.mono_ionic_run.argtypes = [C.c_int, C.c_int, vp, i64, vp, C.c_double, C.c_int]
+    L.mono_model_neq.argtypes = [C.c_int]
+    L.mono_model_initial_state.argtypes = [C.c_int, vp]
+    L.mono_model_initial_v.argtypes = [C.c_int]
+    L.mono_model_initial_v.restype = C.c_double
+    L.mono_assemble.argtypes = [C.POINTER(TissueDesc), vp, dp]
+    L.mono_cg_stencil.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                  C.c_int, C.POINTER(C.c_int), dp]
+    L.mono_cg_csr.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str) -> int:
+    if rc < 0:
+        msg = lib().mono_last_error()
+        raise MonoError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+    return rc
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data

@@ -1,0 +1,73 @@
+// Conjugate-gradient solver for the diffusion step (host interface of cg.hip).
+//
+// Replaces SOLVE_LINEAR_SYSTEM / INIT_LINEAR_SYSTEM (linear_system_solver.c:38-231,
+// gpu_solvers_cublas_12.c:46-232).  All vectors, scalars and the convergence
+// test stay on the device; the host only enqueues kernels and polls a small
+// status record every few iterations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mono_common.h"
+#include "operator.h"
+
+namespace mono {
+
+struct CsrMatrix {
+    int n = 0;
+    int64_t nnz = 0;
+    const int64_t *row_ptr = nullptr;  // device
+    const int32_t *col = nullptr;      // device
+    const double *val = nullptr;       // device
+    const double *diag = nullptr;      // device, optional (Jacobi)
+};
+
+// Stopping rule: the reference's absolute test on r.r (linear_system_solver.c:95,159-163)
+// or a relative one (r.r <= rel^2 * b.b).
+struct CgConfig {
+    double tol = 1e-16;      // absolute r.r threshold (reference semantics)
+    double rel_tol = 0.0;    // if > 0: stop on ||r|| <= rel_tol * ||b|| instead
+    int max_its = 200;
+    int jacobi = 0;
+    // mixed precision (FP32 inner CG, FP64 outer residual and correction)
+    int mixed = 0;
+    double inner_rel = 1e-4;  // inner stop: ||r_in|| <= inner_rel * ||r_in0||
+    int max_outer = 20;
+    int poll_chunk = 0;       // iterations enqueued between status polls (0 = adaptive)
+};
+
+struct CgStats {
+    int iterations = 0;       // as the reference counts them (1 + CG iterations)
+    double error = 0.0;       // final r.r (FP64)
+    int outer = 0;            // mixed: outer sweeps
+    int inner_total = 0;      // mixed: total inner iterations
+};
+
+class CgWorkspace {
+  public:
+    int init(int64_t n, int64_t pitch_hint);
+    int release();
+    // Solve A x = b in FP64 (x is the warm start and the result).
+    int solve_fp64(const OpView<double> &op, const double *b, double *x, const CgConfig &cfg, CgStats *st, hipStream_t s);
+    // Mixed precision: FP64 outer defect correction around an FP32 CG.
+    int solve_mixed(const OpView<double> &op64, const OpView<float> &op32, const double *b, double *x, const CgConfig &cfg,
+                    CgStats *st, hipStream_t s);
+    // CSR variant (non-uniform meshes), FP64.
+    int solve_csr(const CsrMatrix &A, const double *b, double *x, const CgConfig &cfg, CgStats *st, hipStream_t s);
+
+    int64_t n = 0;
+    // FP64 vectors
+    double *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
+    // FP32 vectors (mixed inner solve)
+    float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *d32 = nullptr;
+    double *partials = nullptr;     // per-block partial sums
+    unsigned *counter = nullptr;    // last-block ticket
+    CgScalars *sc = nullptr;        // device scalars (outer / FP64 solve)
+    CgScalars *sc_in = nullptr;     // device scalars (inner solve)
+    CgScalars *host_sc = nullptr;   // pinned mirror for polling
+    int last_iters = 0;
+    int last_inner = 0;
+};
+
+}  // namespace mono

@@ -1,0 +1,151 @@
+// Ionic step kernels: one lane per cell, structure-of-arrays state in HBM.
+//
+// Replaces SOLVE_MODEL_ODES / SET_ODE_INITIAL_CONDITIONS_GPU of the reference
+// (ten_tusscher_2006_RS_GPU.cu:8-156, Ohara_Rudy_2011.cu:8-139,
+// mitchell_shaeffer_2003.cu:7-128) and fuses UPDATE_MONODOMAIN
+// (update_monodomain.c:19-56) into the epilogue: the kernel reads the FP64
+// potential v written by the previous diffusion solve (the Vm mirror of
+// monodomain_solver.c:1012-1047 becomes a plain read), advances all states by
+// num_steps ODE steps, and writes the right-hand side b = alpha * V* for the
+// next solve.  The stimulus is evaluated on the fly from the cell centre and a
+// per-step amplitude computed on the host (ode_solver.c:288-341).
+//
+// State layout: sv[s * pitch + cell], s = 0..neq-1, in T (double or float).
+// Row 0 (V) is only written when `write_vm_row` is set (the reference's plugin
+// contract that row 0 holds Vm after SOLVE_MODEL_ODES).
+#include "ionic.h"
+#include "ionic_models.h"
+#include "mono_common.h"
+
+namespace mono {
+
+template <int MODEL> struct ModelTraits;
+template <> struct ModelTraits<kTT2006> { static constexpr int neq = 19; };
+template <> struct ModelTraits<kORd2011> { static constexpr int neq = 41; };
+template <> struct ModelTraits<kMS2003> { static constexpr int neq = 2; };
+
+template <int MODEL, typename T>
+__device__ __forceinline__ T model_step(T *y, T stim, T dt, const float *scale) {
+    if constexpr(MODEL == kTT2006) {
+        return tt2006_step<T>(y, stim, dt, T(scale[0]), T(scale[1]), T(scale[2]));
+    } else if constexpr(MODEL == kORd2011) {
+        return ord2011_step<T>(y, stim, dt);
+    } else {
+        return ms2003_step<T>(y, stim, dt, T(scale[0]));
+    }
+}
+
+template <int MODEL, typename T>
+__global__ __launch_bounds__(256) void k_ionic_step(IonicLaunch<T> a) {
+    constexpr int NEQ = ModelTraits<MODEL>::neq;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= a.n) return;
+    if(a.active && !a.active[i]) return;
+
+    T y[NEQ];
+#pragma unroll
+    for(int s = 1; s < NEQ; s++) y[s] = a.sv[(size_t)s * a.pitch + i];
+    double V = a.first_step ? (double)a.sv[i] : a.v[i];
+
+    // stimulus for this cell: per-cell array, else x-limit mask on the centre
+    double st = 0.0;
+    if(a.stim_cell) {
+        st = a.stim_cell[i];
+    } else if(a.stim_amp != 0.0) {
+        const int ix = (int)(i % a.nx);
+        const double cx = ix * a.dx + a.dx / 2.0;
+        if(cx < a.stim_x_limit) st = a.stim_amp;
+    }
+    float scale[4] = {1.f, 1.f, 1.f, 1.f};
+    if(a.scale) {
+#pragma unroll
+        for(int k = 0; k < 4; k++) scale[k] = a.scale[4 * i + k];
+    }
+    for(int k = 0; k < a.num_steps; k++) {
+        y[0] = T(V);
+        const T dv = model_step<MODEL, T>(y, T(st), a.dt, scale);
+        V = V + (double)dv;
+    }
+#pragma unroll
+    for(int s = 1; s < NEQ; s++) a.sv[(size_t)s * a.pitch + i] = y[s];
+    if(a.write_vm_row) a.sv[i] = T(V);
+    // fused UPDATE_MONODOMAIN: b = Vm * alpha
+    if(a.b) a.b[i] = V * a.alpha;
+}
+
+template <int MODEL, typename T>
+__global__ __launch_bounds__(256) void k_ionic_init(T *sv, size_t pitch, double *v, int64_t n, const T *ic) {
+    constexpr int NEQ = ModelTraits<MODEL>::neq;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+#pragma unroll
+    for(int s = 0; s < NEQ; s++) sv[(size_t)s * pitch + i] = ic[s];
+    if(v) v[i] = (double)ic[0];
+}
+
+// Default initial conditions (ten_tusscher_2006_RS_CPU.c:28-46, Ohara_Rudy_2011.c:25-65,
+// mitchell_shaeffer_2003.c:24-25); float literals where the reference has them.
+void default_initial_state(int model, double *out) {
+    if(model == kTT2006) {
+        static const float ic[19] = {-85.23f, 0.00621f, 0.4712f, 0.0095f, 0.00172f, 0.7444f, 0.7045f,
+                                     3.373e-5f, 0.7888f, 0.9755f, 0.9953f, 0.999998f, 2.42e-8f, 0.000126f,
+                                     3.64f, 0.00036f, 0.9073f, 8.604f, 136.89f};
+        for(int s = 0; s < 19; s++) out[s] = (double)ic[s];
+    } else if(model == kORd2011) {
+        static const double ic[41] = {-85.f, 7, 7, 145, 145, 1.0e-4, 1.0e-4, 1.2, 1.2, 0, 1, 1, 1, 1,
+                                      1, 0, 1, 1, 0, 1, 1, 0, 1, 1, 0, 1, 1, 1, 1, 1, 0, 1, 1, 0, 0, 0,
+                                      0, 1, 0, 0, 0};
+        for(int s = 0; s < 41; s++) out[s] = ic[s];
+    } else {
+        out[0] = (double)0.00000820413566106744f;
+        out[1] = (double)0.8789655121804799f;
+    }
+}
+
+// INITIAL_V of the model headers: the PDE potential's initial value
+// (ten_tusscher_2006.h:7, Ohara_Rudy_2011.h:8, mitchell_shaeffer_2003.h:8)
+double model_initial_v(int model) {
+    if(model == kTT2006) return (double)(-85.23f);
+    if(model == kORd2011) return (double)(-85.f);
+    return 0.00000820413566106744;
+}
+
+template <typename T>
+static hipError_t launch_step_t(int model, const IonicLaunch<T> &a, hipStream_t s) {
+    const int block = 256;
+    const unsigned grid = (unsigned)((a.n + block - 1) / block);
+    if(a.n == 0) return hipSuccess;
+    switch(model) {
+    case kTT2006: k_ionic_step<kTT2006, T><<<grid, block, 0, s>>>(a); break;
+    case kORd2011: k_ionic_step<kORd2011, T><<<grid, block, 0, s>>>(a); break;
+    case kMS2003: k_ionic_step<kMS2003, T><<<grid, block, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ionic_step(int model, const IonicLaunch<double> &a, hipStream_t s) { return launch_step_t(model, a, s); }
+hipError_t launch_ionic_step(int model, const IonicLaunch<float> &a, hipStream_t s) { return launch_step_t(model, a, s); }
+
+template <typename T>
+static hipError_t launch_init_t(int model, T *sv, size_t pitch, double *v, int64_t n, const T *ic_dev, hipStream_t s) {
+    const int block = 256;
+    const unsigned grid = (unsigned)((n + block - 1) / block);
+    if(n == 0) return hipSuccess;
+    switch(model) {
+    case kTT2006: k_ionic_init<kTT2006, T><<<grid, block, 0, s>>>(sv, pitch, v, n, ic_dev); break;
+    case kORd2011: k_ionic_init<kORd2011, T><<<grid, block, 0, s>>>(sv, pitch, v, n, ic_dev); break;
+    case kMS2003: k_ionic_init<kMS2003, T><<<grid, block, 0, s>>>(sv, pitch, v, n, ic_dev); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ionic_init(int model, double *sv, size_t pitch, double *v, int64_t n, const double *ic, hipStream_t s) {
+    return launch_init_t(model, sv, pitch, v, n, ic, s);
+}
+hipError_t launch_ionic_init(int model, float *sv, size_t pitch, double *v, int64_t n, const float *ic, hipStream_t s) {
+    return launch_init_t(model, sv, pitch, v, n, ic, s);
+}
+
+}  // namespace mono

@@ -1,0 +1,91 @@
+// Finite-volume diffusion operator on a structured box (host interface).
+//
+// The operator replaces the per-cell `elements` rows that ASSEMBLY_MATRIX
+// builds in the reference (matrix_assembly.c:334-514) and the CSR that
+// INIT_LINEAR_SYSTEM uploads (grid.c:573-649).  Rows are first assembled on the
+// device as 27 FP64 coefficients per cell; the operator is then stored in one
+// of two forms:
+//   - class table: when every row equals the row of its geometric class
+//     (z layer x boundary position in x and y), only nz*9 rows are kept and the
+//     kernels index them on the fly (homogeneous or layered tissue: zero bytes
+//     of coefficient traffic per iteration);
+//   - per-cell: otherwise (fibrosis, heterogeneous conductivity), the non-zero
+//     slots are stored as structure-of-arrays [slot][cell].
+// Both carry an FP64 and an FP32 copy (the FP32 one for the inner solver of
+// the mixed-precision path).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mono {
+
+constexpr int kMaxSlots = 27;
+
+// how the conductivity tensor of a cell is given
+enum TensorSource : int {
+    kTensorUniform = 0,   // one (sxx, syy, szz, sxy, sxz, syz) for all cells
+    kTensorPerLayer = 1,  // one tensor per z layer: [nz][6]
+    kTensorPerCell = 2,   // [n][6]
+};
+
+struct AssemblyParams {
+    int nx, ny, nz;
+    double dx, dy, dz;
+    double alpha;
+    int aniso;                // 0: isotropic 7-point rule, 1: anisotropic rule
+    int tensor_source;        // TensorSource
+    const double *tensor;     // device pointer, layout per tensor_source
+    const double *g;          // optional per-cell conductivity factor (device)
+    const uint8_t *active;    // optional per-cell 0/1 (device)
+};
+
+// Device view of an operator used by the kernels.  Slot lists are padded to
+// the kernel's compile-time size with zero coefficients at offset 0, so row
+// products run without data-dependent branches.
+template <typename T> struct OpView {
+    int nx, ny, nz;
+    int64_t n;
+    int nslots;                 // padded slot count S (7, 19 or 27)
+    int nint;                   // padded interior slot count NI (class mode)
+    int8_t off[kMaxSlots][3];   // neighbour offsets of the stored slots (slot 0 = diagonal)
+    int32_t lin[kMaxSlots];     // the same offsets as cell-index differences
+    int32_t lin_int[kMaxSlots]; // interior slot list (class mode)
+    const T *table;             // class mode: [(z*9 + cy*3 + cx) * S + k], else null
+    const T *table_int;         // class mode: [z * NI + k] (interior rows), else null
+    const T *percell;           // per-cell mode: [k * n + cell], else null
+    const uint8_t *active;      // optional
+};
+
+// smallest kernel slot count that holds k slots
+inline int padded_slots(int k) { return k <= 7 ? 7 : (k <= 19 ? 19 : 27); }
+
+struct Operator {
+    int nx = 0, ny = 0, nz = 0;
+    int64_t n = 0;
+    int nslots = 0;        // distinct stored slots (union over all rows)
+    int S = 0;             // padded
+    int nint = 0, NI = 0;  // interior rows' slot union and its padded size
+    int8_t off[kMaxSlots][3] = {};
+    int8_t off_int[kMaxSlots][3] = {};
+    bool class_mode = false;
+    double *table64 = nullptr, *tint64 = nullptr;
+    float *table32 = nullptr, *tint32 = nullptr;
+    double *percell64 = nullptr;
+    float *percell32 = nullptr;
+    const uint8_t *active = nullptr;  // not owned
+    double alpha = 0.0;
+
+    OpView<double> view64() const;
+    OpView<float> view32() const;
+    int release();
+};
+
+// Assemble rows on the device into coef27 ([27][n], FP64, caller-allocated).
+hipError_t launch_assemble(const AssemblyParams &p, double *coef27, hipStream_t s);
+
+// Build an Operator from assembled rows (coef27 is consumed but not freed).
+// Returns 0 on success, -1 on a HIP error (message set).
+int build_operator(const AssemblyParams &p, const double *coef27, Operator *op, hipStream_t s);
+
+}  // namespace mono

@@ -1,0 +1,676 @@
+// Device-resident operator-split tissue run and the C ABI (monoalg_hip.h).
+//
+// One loop iteration follows solve_monodomain (monodomain_solver.c:616-925):
+//   stimulus window test on the accumulated double time (ode_solver.c:319-336)
+//   -> ionic step with the fused RHS update (ionic.hip)
+//   -> diffusion solve with warm start from the previous potential (cg.hip)
+//   -> count++, time += dt_pde.
+// Nothing crosses PCIe inside the loop except the solver's small status record.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/monoalg_hip.h"
+#include "cg.h"
+#include "ionic.h"
+#include "mono_common.h"
+#include "operator.h"
+
+static thread_local char g_err[1024];
+
+void mono_set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+using namespace mono;
+
+struct mono_tissue {
+    mono_tissue_desc d{};
+    int64_t n = 0;
+    int neq = 0;
+    size_t pitch = 0;
+    void *sv = nullptr;  // double* or float*
+    double *v = nullptr;
+    double *b = nullptr;
+    uint8_t *active = nullptr;
+    double *g = nullptr;
+    double *tensor = nullptr;
+    Operator op;
+    CgWorkspace cg;
+    CgConfig cgcfg;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    int timing = 0;
+    int num_ode_steps = 1;
+    double dt_ode = 0.02;
+    double alpha = 0.0;
+    double stim_start = 0.0;
+    double cur_time = 0.0;
+    mono_stats st{};
+};
+
+static double alpha_of(const mono_tissue_desc &d) {
+    // ALPHA (monodomain/constants.h:10-11); the 1e-8 factor is a float literal there
+    return (((d.beta * d.cm) / d.dt_pde) * 0.00000001f) * ((d.dx * d.dy) * d.dz);
+}
+
+static int validate(const mono_tissue_desc *d) {
+    if(!d) {
+        mono_set_error("null descriptor");
+        return -1;
+    }
+    if(d->nx <= 0 || d->ny <= 0 || d->nz <= 0) {
+        mono_set_error("box must have at least one cell per axis (got %d x %d x %d)", d->nx, d->ny, d->nz);
+        return -1;
+    }
+    if((int64_t)d->nx * d->ny * d->nz > (int64_t)1 << 31) {
+        mono_set_error("too many cells for one tissue: %lld", (long long)d->nx * d->ny * d->nz);
+        return -1;
+    }
+    if(!(d->dx > 0) || !(d->dy > 0) || !(d->dz > 0) || !(d->dt_pde > 0)) {
+        mono_set_error("dx, dy, dz and dt_pde must be positive");
+        return -1;
+    }
+    if(d->model < 0 || d->model > 2) {
+        mono_set_error("unknown ionic model %d", d->model);
+        return -1;
+    }
+    if(d->precision != MONO_FP64 && d->precision != MONO_FP32_REFINE) {
+        mono_set_error("unknown precision %d", d->precision);
+        return -1;
+    }
+    if(!d->tensor) {
+        mono_set_error("conductivity tensor missing");
+        return -1;
+    }
+    if(d->tensor_source < 0 || d->tensor_source > 2) {
+        mono_set_error("unknown tensor source %d", d->tensor_source);
+        return -1;
+    }
+    return 0;
+}
+
+static size_t tensor_count(const mono_tissue_desc &d, int64_t n) {
+    return d.tensor_source == MONO_TENSOR_UNIFORM ? 6 : (d.tensor_source == MONO_TENSOR_PER_LAYER ? 6 * (size_t)d.nz : 6 * (size_t)n);
+}
+
+// Upload the optional per-cell inputs and assemble; coef27 is left allocated on return.
+static int assemble_into(const mono_tissue_desc &d, int64_t n, double **tensor_dev, double **g_dev, uint8_t **act_dev,
+                         double **coef27, AssemblyParams *ap, hipStream_t s) {
+    const size_t nt = tensor_count(d, n);
+    MONO_CHECK(hipMalloc(tensor_dev, nt * sizeof(double)));
+    MONO_CHECK(hipMemcpyAsync(*tensor_dev, d.tensor, nt * sizeof(double), hipMemcpyHostToDevice, s));
+    if(d.g_cell) {
+        MONO_CHECK(hipMalloc(g_dev, n * sizeof(double)));
+        MONO_CHECK(hipMemcpyAsync(*g_dev, d.g_cell, n * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    if(d.active) {
+        MONO_CHECK(hipMalloc(act_dev, n));
+        MONO_CHECK(hipMemcpyAsync(*act_dev, d.active, n, hipMemcpyHostToDevice, s));
+    }
+    MONO_CHECK(hipMalloc(coef27, 27 * n * sizeof(double)));
+    ap->nx = d.nx;
+    ap->ny = d.ny;
+    ap->nz = d.nz;
+    ap->dx = d.dx;
+    ap->dy = d.dy;
+    ap->dz = d.dz;
+    ap->alpha = alpha_of(d);
+    ap->aniso = d.aniso;
+    ap->tensor_source = d.tensor_source;
+    ap->tensor = *tensor_dev;
+    ap->g = *g_dev;
+    ap->active = *act_dev;
+    MONO_CHECK(launch_assemble(*ap, *coef27, s));
+    return 0;
+}
+
+extern "C" {
+
+const char *mono_last_error(void) { return g_err; }
+
+int mono_device_count(void) {
+    int c = 0;
+    if(hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int mono_model_neq(int model) { return (model >= 0 && model <= 2) ? model_neq(model) : -1; }
+
+int mono_model_initial_state(int model, double *out) {
+    if(model < 0 || model > 2 || !out) {
+        mono_set_error("bad model or output");
+        return -1;
+    }
+    default_initial_state(model, out);
+    return 0;
+}
+
+double mono_model_initial_v(int model) { return model_initial_v(model); }
+
+int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out) {
+    if(validate(desc)) return -1;
+    if(mono_device_count() < 1) {
+        mono_set_error("no HIP device available");
+        return -2;
+    }
+    mono_tissue *t = new mono_tissue();
+    t->d = *desc;
+    t->d.tensor = nullptr;
+    t->d.g_cell = nullptr;
+    t->d.active = nullptr;
+    const mono_tissue_desc &d = *desc;
+    t->n = (int64_t)d.nx * d.ny * d.nz;
+    t->neq = model_neq(d.model);
+    t->pitch = (size_t)((t->n + 63) / 64 * 64);
+    t->alpha = alpha_of(d);
+    t->dt_ode = d.dt_ode > 0 ? d.dt_ode : d.dt_pde;
+    t->num_ode_steps = d.dt_pde >= t->dt_ode ? (int)(d.dt_pde / t->dt_ode) : 1;
+    t->stim_start = d.stim_start;
+    t->cgcfg.tol = d.cg_tol;
+    t->cgcfg.rel_tol = d.cg_rel_tol;
+    t->cgcfg.max_its = d.cg_max_its > 0 ? d.cg_max_its : 200;
+    t->cgcfg.jacobi = d.cg_jacobi;
+    t->cgcfg.mixed = d.precision == MONO_FP32_REFINE;
+    t->cgcfg.inner_rel = d.inner_rel > 0 ? d.inner_rel : 1e-4;
+    if(t->cgcfg.mixed && t->cgcfg.jacobi) {
+        mono_set_error("FP32_REFINE does not take the Jacobi preconditioner");
+        delete t;
+        return -1;
+    }
+    hipStream_t s;
+    if(hipStreamCreate(&s) != hipSuccess) {
+        mono_set_error("hipStreamCreate failed");
+        delete t;
+        return -1;
+    }
+    t->stream = s;
+    for(int k = 0; k < 3; k++)
+        if(hipEventCreate(&t->ev[k]) != hipSuccess) {
+            mono_set_error("hipEventCreate failed");
+            mono_tissue_destroy(t);
+            return -1;
+        }
+    auto fail = [&](void) {
+        mono_tissue_destroy(t);
+        return -1;
+    };
+
+    // operator
+    double *coef27 = nullptr;
+    AssemblyParams ap{};
+    if(assemble_into(d, t->n, &t->tensor, &t->g, &t->active, &coef27, &ap, s)) return fail();
+    if(build_operator(ap, coef27, &t->op, s)) return fail();
+    if(hipFree(coef27) != hipSuccess) {
+        mono_set_error("hipFree failed");
+        return fail();
+    }
+    t->st.class_operator = t->op.class_mode;
+    t->st.stencil_points = t->op.nslots;
+
+    // state, potential, rhs
+    const size_t wsv = d.precision == MONO_FP64 ? sizeof(double) : sizeof(float);
+    if(hipMalloc(&t->sv, wsv * t->pitch * t->neq) != hipSuccess || hipMalloc(&t->v, sizeof(double) * t->n) != hipSuccess ||
+       hipMalloc(&t->b, sizeof(double) * t->n) != hipSuccess) {
+        mono_set_error("out of device memory for %lld cells", (long long)t->n);
+        return fail();
+    }
+    if(hipMemsetAsync(t->v, 0, sizeof(double) * t->n, s) != hipSuccess) return fail();
+    double ic[64];
+    default_initial_state(d.model, ic);
+    hipError_t e;
+    if(d.precision == MONO_FP64) {
+        double *dic = nullptr;
+        if(hipMalloc(&dic, sizeof(double) * t->neq) != hipSuccess ||
+           hipMemcpy(dic, ic, sizeof(double) * t->neq, hipMemcpyHostToDevice) != hipSuccess)
+            return fail();
+        e = launch_ionic_init(d.model, (double *)t->sv, t->pitch, nullptr, t->n, dic, s);
+        if(e == hipSuccess) e = hipStreamSynchronize(s);
+        if(hipFree(dic) != hipSuccess) return fail();
+    } else {
+        float icf[64];
+        for(int k = 0; k < t->neq; k++) icf[k] = (float)ic[k];
+        float *dic = nullptr;
+        if(hipMalloc(&dic, sizeof(float) * t->neq) != hipSuccess ||
+           hipMemcpy(dic, icf, sizeof(float) * t->neq, hipMemcpyHostToDevice) != hipSuccess)
+            return fail();
+        e = launch_ionic_init(d.model, (float *)t->sv, t->pitch, nullptr, t->n, dic, s);
+        if(e == hipSuccess) e = hipStreamSynchronize(s);
+        if(hipFree(dic) != hipSuccess) return fail();
+    }
+    if(e != hipSuccess) {
+        mono_set_error("initial-state kernel: %s", hipGetErrorString(e));
+        return fail();
+    }
+    // set_initial_conditions_fvm: v = INITIAL_V on active cells (0 elsewhere)
+    {
+        std::vector<double> hv(t->n, 0.0);
+        const double v0 = model_initial_v(d.model);
+        for(int64_t i = 0; i < t->n; i++) hv[i] = (!desc->active || desc->active[i]) ? v0 : 0.0;
+        if(hipMemcpy(t->v, hv.data(), sizeof(double) * t->n, hipMemcpyHostToDevice) != hipSuccess) return fail();
+        for(int64_t i = 0; i < t->n; i++) hv[i] *= t->alpha;
+        if(hipMemcpy(t->b, hv.data(), sizeof(double) * t->n, hipMemcpyHostToDevice) != hipSuccess) return fail();
+    }
+    if(t->cg.init(t->n, t->pitch)) return fail();
+    *out = t;
+    return 0;
+}
+
+int mono_tissue_destroy(mono_tissue *t) {
+    if(!t) return 0;
+    int rc = 0;
+    if(t->stream && hipStreamSynchronize(t->stream) != hipSuccess) rc = -1;
+    if(t->cg.release()) rc = -1;
+    if(t->op.release()) rc = -1;
+    void *bufs[] = {t->sv, t->v, t->b, t->active, t->g, t->tensor};
+    for(void *p : bufs)
+        if(p && hipFree(p) != hipSuccess) rc = -1;
+    for(int k = 0; k < 3; k++)
+        if(t->ev[k] && hipEventDestroy(t->ev[k]) != hipSuccess) rc = -1;
+    if(t->stream && hipStreamDestroy(t->stream) != hipSuccess) rc = -1;
+    delete t;
+    return rc;
+}
+
+int64_t mono_tissue_num_cells(const mono_tissue *t) { return t ? t->n : -1; }
+int mono_tissue_neq(const mono_tissue *t) { return t ? t->neq : -1; }
+
+int mono_tissue_set_timing(mono_tissue *t, int on) {
+    if(!t) return -1;
+    t->timing = on;
+    return 0;
+}
+
+static int launch_ionic(mono_tissue *t, double stim_amp) {
+    const mono_tissue_desc &d = t->d;
+    if(d.precision == MONO_FP64) {
+        IonicLaunch<double> a{};
+        a.sv = (double *)t->sv;
+        a.pitch = t->pitch;
+        a.v = t->v;
+        a.b = t->b;
+        a.n = t->n;
+        a.active = t->active;
+        a.nx = d.nx;
+        a.dx = d.dx;
+        a.stim_x_limit = d.stim_x_limit;
+        a.stim_amp = stim_amp;
+        a.alpha = t->alpha;
+        a.dt = t->dt_ode;
+        a.num_steps = t->num_ode_steps;
+        a.first_step = t->st.steps == 0;
+        MONO_CHECK(launch_ionic_step(d.model, a, t->stream));
+    } else {
+        IonicLaunch<float> a{};
+        a.sv = (float *)t->sv;
+        a.pitch = t->pitch;
+        a.v = t->v;
+        a.b = t->b;
+        a.n = t->n;
+        a.active = t->active;
+        a.nx = d.nx;
+        a.dx = d.dx;
+        a.stim_x_limit = d.stim_x_limit;
+        a.stim_amp = stim_amp;
+        a.alpha = t->alpha;
+        a.dt = (float)t->dt_ode;
+        a.num_steps = t->num_ode_steps;
+        a.first_step = t->st.steps == 0;
+        MONO_CHECK(launch_ionic_step(d.model, a, t->stream));
+    }
+    return 0;
+}
+
+int mono_tissue_step(mono_tissue *t, int nsteps) {
+    if(!t) {
+        mono_set_error("null tissue");
+        return -1;
+    }
+    const mono_tissue_desc &d = t->d;
+    for(int k = 0; k < nsteps; k++) {
+        // merged stimulus (ode_solver.c:288-341): inclusive window on the
+        // accumulated time, summed over the ODE sub-steps
+        double amp = 0.0;
+        double tt = t->cur_time;
+        for(int j = 0; j < t->num_ode_steps; j++) {
+            if((tt >= t->stim_start) && (tt <= t->stim_start + d.stim_duration)) amp += d.stim_current;
+            tt += t->dt_ode;
+        }
+        if(d.stim_period > 0.0 && tt >= t->stim_start + d.stim_period) t->stim_start = t->stim_start + d.stim_period;
+
+        if(t->timing) MONO_CHECK(hipEventRecord(t->ev[0], t->stream));
+        if(launch_ionic(t, amp)) return -1;
+        if(t->timing) MONO_CHECK(hipEventRecord(t->ev[1], t->stream));
+        CgStats cs{};
+        int rc;
+        if(d.precision == MONO_FP64) rc = t->cg.solve_fp64(t->op.view64(), t->b, t->v, t->cgcfg, &cs, t->stream);
+        else rc = t->cg.solve_mixed(t->op.view64(), t->op.view32(), t->b, t->v, t->cgcfg, &cs, t->stream);
+        if(rc) return -1;
+        if(t->timing) {
+            MONO_CHECK(hipEventRecord(t->ev[2], t->stream));
+            MONO_CHECK(hipEventSynchronize(t->ev[2]));
+            float a = 0, b = 0;
+            MONO_CHECK(hipEventElapsedTime(&a, t->ev[0], t->ev[1]));
+            MONO_CHECK(hipEventElapsedTime(&b, t->ev[1], t->ev[2]));
+            t->st.ionic_ms_total += a;
+            t->st.solve_ms_total += b;
+        }
+        t->st.steps++;
+        t->st.cg_iterations_total += cs.iterations;
+        t->st.last_cg_iterations = cs.iterations;
+        t->st.last_cg_error = cs.error;
+        t->st.last_outer = cs.outer;
+        t->st.inner_iterations_total += cs.inner_total;
+        t->st.outer_sweeps_total += cs.outer;
+        t->cur_time += d.dt_pde;
+        t->st.time = t->cur_time;
+        if(isnan(cs.error)) {
+            t->st.nan_detected = 1;
+            return 1;
+        }
+    }
+    return 0;
+}
+
+int mono_tissue_sync(mono_tissue *t) {
+    if(!t) return -1;
+    MONO_CHECK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+int mono_tissue_get_v(mono_tissue *t, double *host_v) {
+    if(!t || !host_v) return -1;
+    MONO_CHECK(hipMemcpyAsync(host_v, t->v, sizeof(double) * t->n, hipMemcpyDeviceToHost, t->stream));
+    MONO_CHECK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+int mono_tissue_set_v(mono_tissue *t, const double *host_v) {
+    if(!t || !host_v) return -1;
+    MONO_CHECK(hipMemcpyAsync(t->v, host_v, sizeof(double) * t->n, hipMemcpyHostToDevice, t->stream));
+    MONO_CHECK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+int mono_tissue_get_state(mono_tissue *t, double *host_sv) {
+    if(!t || !host_sv) return -1;
+    const size_t w = t->d.precision == MONO_FP64 ? sizeof(double) : sizeof(float);
+    std::vector<char> tmp(w * t->pitch * t->neq);
+    MONO_CHECK(hipMemcpyAsync(tmp.data(), t->sv, tmp.size(), hipMemcpyDeviceToHost, t->stream));
+    MONO_CHECK(hipStreamSynchronize(t->stream));
+    for(int s = 0; s < t->neq; s++)
+        for(int64_t i = 0; i < t->n; i++) {
+            const size_t k = (size_t)s * t->pitch + i;
+            host_sv[(size_t)s * t->n + i] = w == 8 ? ((double *)tmp.data())[k] : (double)((float *)tmp.data())[k];
+        }
+    return 0;
+}
+
+int mono_tissue_set_state(mono_tissue *t, const double *host_sv) {
+    if(!t || !host_sv) return -1;
+    const size_t w = t->d.precision == MONO_FP64 ? sizeof(double) : sizeof(float);
+    std::vector<char> tmp(w * t->pitch * t->neq, 0);
+    for(int s = 0; s < t->neq; s++)
+        for(int64_t i = 0; i < t->n; i++) {
+            const size_t k = (size_t)s * t->pitch + i;
+            if(w == 8) ((double *)tmp.data())[k] = host_sv[(size_t)s * t->n + i];
+            else ((float *)tmp.data())[k] = (float)host_sv[(size_t)s * t->n + i];
+        }
+    MONO_CHECK(hipMemcpyAsync(t->sv, tmp.data(), tmp.size(), hipMemcpyHostToDevice, t->stream));
+    MONO_CHECK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+int mono_tissue_get_stats(const mono_tissue *t, mono_stats *out) {
+    if(!t || !out) return -1;
+    *out = t->st;
+    return 0;
+}
+
+int mono_tissue_device_ptrs(mono_tissue *t, void **v, void **sv, size_t *pitch, void **b) {
+    if(!t) return -1;
+    if(v) *v = t->v;
+    if(sv) *sv = t->sv;
+    if(pitch) *pitch = t->pitch;
+    if(b) *b = t->b;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// standalone entry points
+// ---------------------------------------------------------------------------
+
+int mono_ionic_run(int model, int precision, double *states, int64_t n, const double *stim, double dt, int nsteps) {
+    if(model < 0 || model > 2 || !states || n < 0 || nsteps < 0) {
+        mono_set_error("mono_ionic_run: bad arguments");
+        return -1;
+    }
+    if(precision != MONO_FP64 && precision != MONO_FP32_REFINE) {
+        mono_set_error("mono_ionic_run: unknown precision %d", precision);
+        return -1;
+    }
+    if(mono_device_count() < 1) {
+        mono_set_error("no HIP device available");
+        return -2;
+    }
+    if(n == 0) return 0;
+    const int neq = model_neq(model);
+    const size_t pitch = (size_t)((n + 63) / 64 * 64);
+    const size_t w = precision == MONO_FP64 ? 8 : 4;
+    std::vector<char> h(w * pitch * neq, 0);
+    for(int s = 0; s < neq; s++)
+        for(int64_t i = 0; i < n; i++) {
+            const double val = states[i * neq + s];
+            if(w == 8) ((double *)h.data())[s * pitch + i] = val;
+            else ((float *)h.data())[s * pitch + i] = (float)val;
+        }
+    void *dsv = nullptr;
+    double *dv = nullptr, *dstim = nullptr;
+    MONO_CHECK(hipMalloc(&dsv, h.size()));
+    MONO_CHECK(hipMalloc(&dv, sizeof(double) * n));
+    MONO_CHECK(hipMemcpy(dsv, h.data(), h.size(), hipMemcpyHostToDevice));
+    if(stim) {
+        MONO_CHECK(hipMalloc(&dstim, sizeof(double) * n));
+        MONO_CHECK(hipMemcpy(dstim, stim, sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    // V travels in FP64 between calls through v, as in the tissue path
+    std::vector<double> hv(n);
+    for(int64_t i = 0; i < n; i++) hv[i] = states[i * neq];
+    MONO_CHECK(hipMemcpy(dv, hv.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    for(int k = 0; k < nsteps; k++) {
+        hipError_t e;
+        if(precision == MONO_FP64) {
+            IonicLaunch<double> a{};
+            a.sv = (double *)dsv;
+            a.pitch = pitch;
+            a.v = dv;
+            a.b = dv;  // b = alpha * V* with alpha = 1 keeps V* in v for the next step
+            a.alpha = 1.0;
+            a.n = n;
+            a.stim_cell = dstim;
+            a.nx = 1;
+            a.dx = 1.0;
+            a.dt = dt;
+            a.num_steps = 1;
+            a.write_vm_row = 1;
+            e = launch_ionic_step(model, a, nullptr);
+        } else {
+            IonicLaunch<float> a{};
+            a.sv = (float *)dsv;
+            a.pitch = pitch;
+            a.v = dv;
+            a.b = dv;
+            a.alpha = 1.0;
+            a.n = n;
+            a.stim_cell = dstim;
+            a.nx = 1;
+            a.dx = 1.0;
+            a.dt = (float)dt;
+            a.num_steps = 1;
+            a.write_vm_row = 1;
+            e = launch_ionic_step(model, a, nullptr);
+        }
+        if(e != hipSuccess) {
+            mono_set_error("ionic kernel launch: %s", hipGetErrorString(e));
+            return -1;
+        }
+    }
+    MONO_CHECK(hipDeviceSynchronize());
+    MONO_CHECK(hipMemcpy(h.data(), dsv, h.size(), hipMemcpyDeviceToHost));
+    MONO_CHECK(hipMemcpy(hv.data(), dv, sizeof(double) * n, hipMemcpyDeviceToHost));
+    for(int s = 0; s < neq; s++)
+        for(int64_t i = 0; i < n; i++)
+            states[i * neq + s] = w == 8 ? ((double *)h.data())[s * pitch + i] : (double)((float *)h.data())[s * pitch + i];
+    for(int64_t i = 0; i < n; i++) states[i * neq] = hv[i];
+    MONO_CHECK(hipFree(dsv));
+    MONO_CHECK(hipFree(dv));
+    if(dstim) MONO_CHECK(hipFree(dstim));
+    return 0;
+}
+
+int mono_assemble(const mono_tissue_desc *desc, double *coef27, double *alpha_out) {
+    if(validate(desc)) return -1;
+    if(!coef27) {
+        mono_set_error("mono_assemble: null output");
+        return -1;
+    }
+    if(mono_device_count() < 1) {
+        mono_set_error("no HIP device available");
+        return -2;
+    }
+    const int64_t n = (int64_t)desc->nx * desc->ny * desc->nz;
+    double *tensor = nullptr, *g = nullptr, *dcoef = nullptr;
+    uint8_t *act = nullptr;
+    AssemblyParams ap{};
+    if(assemble_into(*desc, n, &tensor, &g, &act, &dcoef, &ap, nullptr)) return -1;
+    MONO_CHECK(hipDeviceSynchronize());
+    MONO_CHECK(hipMemcpy(coef27, dcoef, sizeof(double) * 27 * n, hipMemcpyDeviceToHost));
+    if(alpha_out) *alpha_out = ap.alpha;
+    void *bufs[] = {tensor, g, act, dcoef};
+    for(void *p : bufs)
+        if(p) MONO_CHECK(hipFree(p));
+    return 0;
+}
+
+int mono_cg_stencil(int nx, int ny, int nz, const double *coef27, const uint8_t *active, const double *b, double *x, double tol,
+                    double rel_tol, int max_its, int jacobi, int precision, int *iterations, double *error) {
+    if(nx <= 0 || ny <= 0 || nz <= 0 || !coef27 || !b || !x) {
+        mono_set_error("mono_cg_stencil: bad arguments");
+        return -1;
+    }
+    if(mono_device_count() < 1) {
+        mono_set_error("no HIP device available");
+        return -2;
+    }
+    const int64_t n = (int64_t)nx * ny * nz;
+    double *dcoef = nullptr, *db = nullptr, *dx = nullptr;
+    uint8_t *dact = nullptr;
+    MONO_CHECK(hipMalloc(&dcoef, sizeof(double) * 27 * n));
+    MONO_CHECK(hipMalloc(&db, sizeof(double) * n));
+    MONO_CHECK(hipMalloc(&dx, sizeof(double) * n));
+    MONO_CHECK(hipMemcpy(dcoef, coef27, sizeof(double) * 27 * n, hipMemcpyHostToDevice));
+    MONO_CHECK(hipMemcpy(db, b, sizeof(double) * n, hipMemcpyHostToDevice));
+    MONO_CHECK(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    if(active) {
+        MONO_CHECK(hipMalloc(&dact, n));
+        MONO_CHECK(hipMemcpy(dact, active, n, hipMemcpyHostToDevice));
+    }
+    AssemblyParams ap{};
+    ap.nx = nx;
+    ap.ny = ny;
+    ap.nz = nz;
+    ap.active = dact;
+    Operator op;
+    if(build_operator(ap, dcoef, &op, nullptr)) return -1;
+    CgWorkspace ws;
+    if(ws.init(n, n)) return -1;
+    CgConfig cfg;
+    cfg.tol = tol;
+    cfg.rel_tol = rel_tol;
+    cfg.max_its = max_its;
+    cfg.jacobi = jacobi;
+    CgStats st{};
+    int rc = precision == MONO_FP64 ? ws.solve_fp64(op.view64(), db, dx, cfg, &st, nullptr)
+                                    : ws.solve_mixed(op.view64(), op.view32(), db, dx, cfg, &st, nullptr);
+    if(rc) return -1;
+    MONO_CHECK(hipDeviceSynchronize());
+    MONO_CHECK(hipMemcpy(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if(iterations) *iterations = st.iterations;
+    if(error) *error = st.error;
+    if(ws.release() || op.release()) return -1;
+    void *bufs[] = {dcoef, db, dx, dact};
+    for(void *p : bufs)
+        if(p) MONO_CHECK(hipFree(p));
+    return 0;
+}
+
+int mono_cg_csr(int n, const int64_t *row_ptr, const int32_t *col, const double *val, const double *b, double *x, double tol,
+                int max_its, int jacobi, int *iterations, double *error) {
+    if(n <= 0 || !row_ptr || !col || !val || !b || !x) {
+        mono_set_error("mono_cg_csr: bad arguments");
+        return -1;
+    }
+    if(mono_device_count() < 1) {
+        mono_set_error("no HIP device available");
+        return -2;
+    }
+    const int64_t nnz = row_ptr[n];
+    for(int64_t k = 0; k < nnz; k++)
+        if(col[k] < 0 || col[k] >= n) {
+            mono_set_error("mono_cg_csr: column index %d out of range", col[k]);
+            return -1;
+        }
+    std::vector<double> diag(n, 0.0);
+    for(int i = 0; i < n; i++)
+        for(int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++)
+            if(col[k] == i) diag[i] = val[k];
+    int64_t *drp = nullptr;
+    int32_t *dcol = nullptr;
+    double *dval = nullptr, *db = nullptr, *dx = nullptr, *ddiag = nullptr;
+    MONO_CHECK(hipMalloc(&drp, sizeof(int64_t) * (n + 1)));
+    MONO_CHECK(hipMalloc(&dcol, sizeof(int32_t) * (nnz > 0 ? nnz : 1)));
+    MONO_CHECK(hipMalloc(&dval, sizeof(double) * (nnz > 0 ? nnz : 1)));
+    MONO_CHECK(hipMalloc(&db, sizeof(double) * n));
+    MONO_CHECK(hipMalloc(&dx, sizeof(double) * n));
+    MONO_CHECK(hipMalloc(&ddiag, sizeof(double) * n));
+    MONO_CHECK(hipMemcpy(drp, row_ptr, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+    if(nnz > 0) {
+        MONO_CHECK(hipMemcpy(dcol, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+        MONO_CHECK(hipMemcpy(dval, val, sizeof(double) * nnz, hipMemcpyHostToDevice));
+    }
+    MONO_CHECK(hipMemcpy(db, b, sizeof(double) * n, hipMemcpyHostToDevice));
+    MONO_CHECK(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    MONO_CHECK(hipMemcpy(ddiag, diag.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    CsrMatrix A;
+    A.n = n;
+    A.nnz = nnz;
+    A.row_ptr = drp;
+    A.col = dcol;
+    A.val = dval;
+    A.diag = ddiag;
+    CgWorkspace ws;
+    if(ws.init(n, n)) return -1;
+    CgConfig cfg;
+    cfg.tol = tol;
+    cfg.max_its = max_its;
+    cfg.jacobi = jacobi;
+    CgStats st{};
+    if(ws.solve_csr(A, db, dx, cfg, &st, nullptr)) return -1;
+    MONO_CHECK(hipDeviceSynchronize());
+    MONO_CHECK(hipMemcpy(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if(iterations) *iterations = st.iterations;
+    if(error) *error = st.error;
+    if(ws.release()) return -1;
+    void *bufs[] = {drp, dcol, dval, db, dx, ddiag};
+    for(void *p : bufs)
+        if(p) MONO_CHECK(hipFree(p));
+    return 0;
+}
+
+}  // extern "C"

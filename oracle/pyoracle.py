@@ -1,0 +1,212 @@
+"""ctypes view of the oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg, where it is the checker (and the timed CPU port),
+never the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+TT2006, ORD2011, MS2003 = 0, 1, 2
+MODEL_IDS = {"tt2006": TT2006, "ord2011": ORD2011, "ms2003": MS2003}
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u8p = C.c_void_p
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-f", os.path.join(HERE, "Makefile")], check=True,
+                   cwd=os.path.dirname(HERE))
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_model_neq.restype = C.c_int
+        L.orc_model_neq.argtypes = [C.c_int]
+        L.orc_model_initial_v.restype = C.c_double
+        L.orc_model_initial_v.argtypes = [C.c_int]
+        L.orc_model_init.argtypes = [C.c_int, _dp, C.c_int64]
+        L.orc_model_solve.argtypes = [C.c_int, _dp, C.c_int64, C.c_void_p, C.c_double, C.c_int]
+        L.orc_alpha.restype = C.c_double
+        L.orc_alpha.argtypes = [C.c_double] * 6
+        L.orc_assemble_iso.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [_dp, _u8p, _dp]
+        L.orc_assemble_aniso.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [_dp, _u8p, _dp]
+        L.orc_fibre_tensor.argtypes = [_dp, C.c_double, C.c_double, _dp]
+        L.orc_normalise.argtypes = [_dp]
+        L.orc_stencil_apply.argtypes = [C.c_int] * 3 + [_dp, _u8p, _dp, _dp]
+        L.orc_cg_stencil.restype = C.c_int
+        L.orc_cg_stencil.argtypes = [C.c_int] * 3 + [_dp, _u8p, _dp, _dp, C.c_double, C.c_int,
+                                                     C.c_int, C.POINTER(C.c_double)]
+        L.orc_cg_csr.restype = C.c_int
+        L.orc_cg_csr.argtypes = [C.c_int, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"),
+                                 np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"), _dp, _dp, _dp,
+                                 C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.orc_tissue_init.argtypes = [C.c_void_p]
+        L.orc_tissue_step.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _mask_ptr(active):
+    if active is None:
+        return None
+    a = np.ascontiguousarray(active, dtype=np.uint8)
+    return a, a.ctypes.data
+
+
+def neq(model: int) -> int:
+    return lib().orc_model_neq(model)
+
+
+def initial_v(model: int) -> float:
+    return lib().orc_model_initial_v(model)
+
+
+def model_init(model: int, n: int) -> np.ndarray:
+    sv = np.zeros(n * neq(model), dtype=np.float64)
+    lib().orc_model_init(model, sv, n)
+    return sv.reshape(n, neq(model))
+
+
+def model_solve(model: int, sv: np.ndarray, stim: np.ndarray | None, dt: float, num_steps: int = 1) -> None:
+    """In place on an AoS (n, neq) float64 array."""
+    assert sv.dtype == np.float64 and sv.flags.c_contiguous
+    n = sv.shape[0]
+    st = None if stim is None else np.ascontiguousarray(stim, dtype=np.float64)
+    lib().orc_model_solve(model, sv.reshape(-1), n, None if st is None else st.ctypes.data, dt, num_steps)
+
+
+def alpha(beta, cm, dt, dx, dy, dz) -> float:
+    return lib().orc_alpha(beta, cm, dt, dx, dy, dz)
+
+
+def assemble_iso(shape, h, alpha_v, sigma3, active=None) -> np.ndarray:
+    nx, ny, nz = shape
+    dx, dy, dz = h
+    n = nx * ny * nz
+    out = np.zeros(n * 27, dtype=np.float64)
+    keep = _mask_ptr(active)
+    lib().orc_assemble_iso(nx, ny, nz, dx, dy, dz, alpha_v, np.ascontiguousarray(sigma3, np.float64).reshape(-1),
+                           None if keep is None else keep[1], out)
+    return out.reshape(n, 27)
+
+
+def assemble_aniso(shape, h, alpha_v, tensor6, active=None) -> np.ndarray:
+    nx, ny, nz = shape
+    dx, dy, dz = h
+    n = nx * ny * nz
+    out = np.zeros(n * 27, dtype=np.float64)
+    keep = _mask_ptr(active)
+    lib().orc_assemble_aniso(nx, ny, nz, dx, dy, dz, alpha_v, np.ascontiguousarray(tensor6, np.float64).reshape(-1),
+                             None if keep is None else keep[1], out)
+    return out.reshape(n, 27)
+
+
+def normalise(f) -> np.ndarray:
+    v = np.array(f, dtype=np.float64)
+    lib().orc_normalise(v)
+    return v
+
+
+def fibre_tensor(f, sigma_l, sigma_t) -> np.ndarray:
+    out = np.zeros(6, dtype=np.float64)
+    lib().orc_fibre_tensor(np.ascontiguousarray(f, np.float64), sigma_l, sigma_t, out)
+    return out
+
+
+def stencil_apply(shape, coef27, x, active=None) -> np.ndarray:
+    nx, ny, nz = shape
+    y = np.zeros(nx * ny * nz, dtype=np.float64)
+    keep = _mask_ptr(active)
+    lib().orc_stencil_apply(nx, ny, nz, np.ascontiguousarray(coef27, np.float64).reshape(-1),
+                            None if keep is None else keep[1], np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def cg_stencil(shape, coef27, b, x0, tol, max_its=200, jacobi=False, active=None):
+    nx, ny, nz = shape
+    x = np.array(x0, dtype=np.float64, copy=True)
+    err = C.c_double(0)
+    keep = _mask_ptr(active)
+    its = lib().orc_cg_stencil(nx, ny, nz, np.ascontiguousarray(coef27, np.float64).reshape(-1),
+                               None if keep is None else keep[1], np.ascontiguousarray(b, np.float64), x, tol,
+                               max_its, int(jacobi), C.byref(err))
+    return x, its, err.value
+
+
+def cg_csr(row_ptr, col, val, b, x0, tol, max_its=200, jacobi=False):
+    x = np.array(x0, dtype=np.float64, copy=True)
+    err = C.c_double(0)
+    its = lib().orc_cg_csr(len(row_ptr) - 1, np.ascontiguousarray(row_ptr, np.int64),
+                           np.ascontiguousarray(col, np.int32), np.ascontiguousarray(val, np.float64),
+                           np.ascontiguousarray(b, np.float64), x, tol, max_its, int(jacobi), C.byref(err))
+    return x, its, err.value
+
+
+class _Tissue(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
+                ("dz", C.c_double), ("model", C.c_int), ("dt", C.c_double), ("beta", C.c_double),
+                ("cm", C.c_double), ("cg_tol", C.c_double), ("cg_max_its", C.c_int), ("cg_jacobi", C.c_int),
+                ("stim_start", C.c_double), ("stim_duration", C.c_double), ("stim_current", C.c_double),
+                ("stim_x_limit", C.c_double), ("stim_period", C.c_double), ("coef27", C.c_void_p),
+                ("active", C.c_void_p), ("sv", C.c_void_p), ("v", C.c_void_p), ("b", C.c_void_p),
+                ("cur_time", C.c_double), ("count", C.c_int64), ("total_cg_its", C.c_int64),
+                ("last_cg_its", C.c_int), ("last_cg_err", C.c_double)]
+
+
+class Tissue:
+    """The oracle's operator-split run on a structured box (AoS state)."""
+
+    def __init__(self, shape, h, model, coef27, dt=0.02, beta=0.14, cm=1.0, cg_tol=1e-16, cg_max_its=200,
+                 jacobi=False, stim=(0.0, 2.0, -50.0, 500.0, 0.0), active=None):
+        nx, ny, nz = shape
+        n = nx * ny * nz
+        self.shape, self.n, self.model = shape, n, model
+        self.neq = neq(model)
+        self.coef27 = np.ascontiguousarray(coef27, np.float64).reshape(-1)
+        self.active = None if active is None else np.ascontiguousarray(active, np.uint8)
+        self.sv = np.zeros(n * self.neq, np.float64)
+        self.v = np.zeros(n, np.float64)
+        self.b = np.zeros(n, np.float64)
+        t = _Tissue()
+        t.nx, t.ny, t.nz = nx, ny, nz
+        t.dx, t.dy, t.dz = h
+        t.model, t.dt, t.beta, t.cm = model, dt, beta, cm
+        t.cg_tol, t.cg_max_its, t.cg_jacobi = cg_tol, cg_max_its, int(jacobi)
+        t.stim_start, t.stim_duration, t.stim_current, t.stim_x_limit, t.stim_period = stim
+        t.coef27 = self.coef27.ctypes.data
+        t.active = None if self.active is None else self.active.ctypes.data
+        t.sv, t.v, t.b = self.sv.ctypes.data, self.v.ctypes.data, self.b.ctypes.data
+        self._t = t
+        lib().orc_tissue_init(C.byref(t))
+
+    def step(self, k: int = 1) -> None:
+        for _ in range(k):
+            lib().orc_tissue_step(C.byref(self._t))
+
+    @property
+    def time(self) -> float:
+        return self._t.cur_time
+
+    @property
+    def last_cg(self):
+        return self._t.last_cg_its, self._t.last_cg_err
+
+    @property
+    def total_cg_its(self) -> int:
+        return self._t.total_cg_its
