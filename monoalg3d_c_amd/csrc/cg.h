@@ -58,11 +58,10 @@ class CgWorkspace {
 
     int64_t n = 0;
     // FP64 vectors
-    double *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
+    double *r = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr, *z = nullptr;
     // FP32 vectors (mixed inner solve)
-    float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *d32 = nullptr;
-    double *partials = nullptr;     // per-block partial sums
-    unsigned *counter = nullptr;    // last-block ticket
+    float *r32 = nullptr, *p32 = nullptr, *p32b = nullptr, *q32 = nullptr, *d32 = nullptr;
+    double *P0 = nullptr, *P1 = nullptr;  // per-block partial sums (alternating between kernels)
     CgScalars *sc = nullptr;        // device scalars (outer / FP64 solve)
     CgScalars *sc_in = nullptr;     // device scalars (inner solve)
     CgScalars *host_sc = nullptr;   // pinned mirror for polling

@@ -4,11 +4,17 @@
 // (linear_system_solver.c:47-183): r = b - A x0, p = r (or z = D^-1 r), then
 // q = A p, alpha = r.z / p.q, x += alpha p, r -= alpha q, beta = r1.z1 / r.z,
 // iteration count starting at 1, stop when r.r <= tol or the count reaches
-// max_its.  Each reduction is fused into the kernel that produces its
-// operands; partial sums are combined by the last block to finish (fixed order,
-// so results are run-to-run deterministic), which also computes the scalars for
-// the next kernel.  Every kernel reads a device `done` flag first, so the host
-// can enqueue several iterations ahead without synchronising.
+// max_its.
+//
+// Reductions use no atomics: every block of a kernel writes its partial sums,
+// and the next kernel, which needs the scalar anyway, adds the partials in a
+// fixed order in each of its blocks (results are run-to-run deterministic).
+// Block 0 of that kernel records the scalars and the convergence decision in a
+// device CgScalars record; every later kernel of the solve reads its `done`
+// flag first, so the host enqueues iterations ahead and only polls now and
+// then.  (Device-scope atomics on one address from thousands of blocks cost
+// hundreds of microseconds on this part; a second kernel launch would cost a
+// few more.)
 //
 // Mixed precision (FP32 CG with FP64 defect correction): the outer residual
 // r = b - A x is formed in FP64 with the FP64 operator; an FP32 CG solves
@@ -20,13 +26,13 @@
 namespace mono {
 
 constexpr int kBlock = 256;
-constexpr int kMaxBlocks = 2048;
+constexpr int kMaxBlocks = 8192;
 constexpr int kMaxRed = 3;
+constexpr int kElemBlocks = 1024;  // grid of the elementwise kernels
 
 __host__ __device__ __forceinline__ int pos_cls(int i, int len) { return i == 0 ? 0 : (i == len - 1 ? 2 : 1); }
 
-// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous eighth of the chunks so stencil neighbours share its L2.
+// Elementwise kernels: contiguous chunks per block, chunks of one XCD adjacent.
 __device__ __forceinline__ int xcd_chunk(int b, int nblocks) {
     if(nblocks % 8 != 0) return b;
     return (b % 8) * (nblocks / 8) + b / 8;
@@ -47,9 +53,10 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Block-wide sums of up to kMaxRed values; result valid in thread 0.
-template <int NR> __device__ __forceinline__ void block_sum(double (&v)[NR]) {
+// Block-wide sums; the totals are returned to every thread.
+template <int NR> __device__ __forceinline__ void block_sum_all(double (&v)[NR]) {
     __shared__ double lds[NR][kBlock / 64];
+    __shared__ double tot[NR];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for(int k = 0; k < NR; k++) {
@@ -62,136 +69,119 @@ template <int NR> __device__ __forceinline__ void block_sum(double (&v)[NR]) {
         for(int k = 0; k < NR; k++) {
             double s = 0.0;
             for(int i = 0; i < kBlock / 64; i++) s += lds[k][i];
-            v[k] = s;
+            tot[k] = s;
         }
     }
+    __syncthreads();
+#pragma unroll
+    for(int k = 0; k < NR; k++) v[k] = tot[k];
 }
 
-// Publish this block's partial sums; returns true (in every thread) in the
-// last block to finish, whose thread 0 then holds the grid totals in v.
-template <int NR>
-__device__ __forceinline__ bool grid_sum(double (&v)[NR], double *partials, unsigned *counter) {
-    block_sum<NR>(v);
-    __shared__ int is_last;
+// This block's contribution to the kernel's reductions.
+template <int NR> __device__ __forceinline__ void write_partials(double (&v)[NR], double *P) {
+    block_sum_all<NR>(v);
     if(threadIdx.x == 0) {
 #pragma unroll
-        for(int k = 0; k < NR; k++)
-            __hip_atomic_store(&partials[k * kMaxBlocks + blockIdx.x], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (t == gridDim.x - 1);
+        for(int k = 0; k < NR; k++) P[k * kMaxBlocks + blockIdx.x] = v[k];
     }
-    __syncthreads();
-    if(!is_last) return false;
-    // fixed-order combination of the partials
-    double acc[NR];
-#pragma unroll
-    for(int k = 0; k < NR; k++) acc[k] = 0.0;
-    for(int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) {
-#pragma unroll
-        for(int k = 0; k < NR; k++)
-            acc[k] += __hip_atomic_load(&partials[k * kMaxBlocks + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    block_sum<NR>(acc);
-    if(threadIdx.x == 0) {
-#pragma unroll
-        for(int k = 0; k < NR; k++) v[k] = acc[k];
-        *counter = 0u;
-    }
-    return true;
 }
 
-// Stencil walk.  Blocks are dispatched round-robin over the 8 XCDs (block b runs
-// on XCD b % 8).  XCD k owns the y-strip [y0, y1) of every z plane and its
-// blocks walk that strip plane by plane in 256-cell chunks, so the z- and
-// y-neighbours a row product reads were fetched into the same XCD's L2 shortly
-// before.  Cell coordinates advance incrementally (no divisions per cell).
-struct Walk {
-    int x, y, z, y0, y1;
-    int64_t l, end;
+// Totals of the previous kernel's partials (fixed order; every thread gets them).
+template <int NR> __device__ __forceinline__ void sum_partials(const double *__restrict__ P, int nprev, double (&tot)[NR]) {
+#pragma unroll
+    for(int k = 0; k < NR; k++) tot[k] = 0.0;
+    for(int i = threadIdx.x; i < nprev; i += kBlock) {
+#pragma unroll
+        for(int k = 0; k < NR; k++) tot[k] += P[k * kMaxBlocks + i];
+    }
+    block_sum_all<NR>(tot);
+}
+
+// Parameters of one solve passed to its kernels.
+struct CgSetup {
+    double tol, rel_tol, inner_rel;
+    int max_its, jacobi, max_outer, sweep;
 };
 
-__device__ __forceinline__ bool walk_begin(int nx, int ny, int nz, Walk &w) {
-    const int nxcd = (gridDim.x % 8 == 0 && ny >= 8) ? 8 : 1;
-    const int k = blockIdx.x % nxcd, j = blockIdx.x / nxcd, nbk = gridDim.x / nxcd;
-    w.y0 = (int)((int64_t)ny * k / nxcd);
-    w.y1 = (int)((int64_t)ny * (k + 1) / nxcd);
-    const int64_t spp = (int64_t)(w.y1 - w.y0) * nx;
-    const int64_t total = spp * nz;
-    int64_t per = (total + nbk - 1) / nbk;
-    per = (per + kBlock - 1) / kBlock * kBlock;
-    w.l = (int64_t)j * per + threadIdx.x;
-    w.end = (int64_t)(j + 1) * per < total ? (int64_t)(j + 1) * per : total;
-    if(w.l >= w.end) return false;
-    w.z = (int)(w.l / spp);
-    const int64_t off = w.l - (int64_t)w.z * spp;
-    const int yy = (int)(off / nx);
-    w.x = (int)(off - (int64_t)yy * nx);
-    w.y = w.y0 + yy;
-    return true;
-}
-
-__device__ __forceinline__ bool walk_next(int nx, Walk &w) {
-    w.l += kBlock;
-    if(w.l >= w.end) return false;
-    w.x += kBlock;
-    while(w.x >= nx) {
-        w.x -= nx;
-        if(++w.y >= w.y1) {
-            w.y = w.y0;
-            ++w.z;
+// Start of an iteration kernel: settle the reduction of the previous kernel.
+//   mode 0: after the FP64 residual (P0 = r.r, r.z, b.b): tolerance, first test
+//   mode 1: after the update of iteration k-1 (P0 = r.r, r.z): beta, stopping test
+//   mode 2: after the mixed-precision outer residual (P0 = r.r, b.b): outer test
+//           and the inner solve's target
+// Returns false (in every thread) when the solve is finished.
+__device__ __forceinline__ bool iter_prologue(int mode, int k, const double *__restrict__ P0, int nprev, CgScalars *sc, CgScalars *so,
+                                              const CgSetup &cs, double &beta) {
+    if(mode == 1) {
+        // Block 0 of this very kernel may set `done` while other blocks start;
+        // one read per block keeps the decision uniform across its waves.
+        __shared__ int s_done;
+        if(threadIdx.x == 0) s_done = sc->done;
+        __syncthreads();
+        if(s_done) return false;
+    }
+    double t[3];
+    if(mode == 0) {
+        sum_partials<3>(P0, nprev, t);
+    } else {
+        double t2[2];
+        sum_partials<2>(P0, nprev, t2);
+        t[0] = t2[0];
+        t[1] = t2[1];
+        t[2] = 0.0;
+    }
+    bool done;
+    if(mode == 0) {
+        const double rr = t[0], rz = cs.jacobi ? t[1] : t[0], bb = t[2];
+        const double tol_abs = cs.rel_tol > 0.0 ? cs.rel_tol * cs.rel_tol * bb : cs.tol;
+        done = !(rr >= tol_abs) || cs.max_its <= 1;
+        beta = 0.0;
+        if(blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->tol_abs = tol_abs;
+            sc->rr0 = rr;
+            sc->rr_true = rr;
+            sc->rrs[0] = rz;
+            sc->iters = 1;
+            sc->max_its = cs.max_its;
+            sc->beta = 0.0;
+            sc->done = done;
+        }
+    } else if(mode == 1) {
+        const double rr = t[0], rz = cs.jacobi ? t[1] : t[0];
+        const int iters = 1 + k;
+        done = (rr <= sc->tol_abs) || (iters >= sc->max_its) || (rr != rr);
+        beta = rz / sc->rrs[(k - 1) & 1];
+        if(blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->rrs[k & 1] = rz;
+            sc->rr_true = rr;
+            sc->iters = iters;
+            sc->beta = beta;
+            sc->done = done;
+        }
+    } else {
+        const double rr = t[0], bb = t[1];
+        const double tol_o = cs.sweep == 0 ? (cs.rel_tol > 0.0 ? cs.rel_tol * cs.rel_tol * bb : cs.tol) : so->tol_abs;
+        done = !(rr > tol_o) || cs.sweep >= cs.max_outer || (rr != rr);
+        const double t_rel = cs.inner_rel * cs.inner_rel * rr;
+        beta = 0.0;
+        if(blockIdx.x == 0 && threadIdx.x == 0) {
+            if(cs.sweep == 0) {
+                so->tol_abs = tol_o;
+                so->rr0 = rr;
+            }
+            so->rr_true = rr;
+            so->iters = cs.sweep + 1;
+            so->done = done;
+            sc->tol_abs = t_rel > 0.5 * tol_o ? t_rel : 0.5 * tol_o;
+            sc->rr0 = rr;
+            sc->rr_true = rr;
+            sc->rrs[0] = rr;
+            sc->iters = 1;
+            sc->max_its = cs.max_its;
+            sc->beta = 0.0;
+            sc->done = done;
         }
     }
-    return true;
-}
-
-// Axis fast range for the interior row path: 1..L-2, or the only cell when L == 1.
-__device__ __forceinline__ bool in_fast(int i, int L) { return L == 1 ? true : (i > 0 && i < L - 1); }
-
-// y_c = (A x)_c over the stored slots; diagonal first, as the reference forms
-// its row products (linear_system_solver.c:78-85).  Slot loops have a
-// compile-time length (S or NI) and no data-dependent branches, so all the
-// neighbour loads of a row can be in flight together.  When every lane of the
-// wave holds a cell of one z layer inside the fast ranges (the common case),
-// the coefficient row is wave-uniform; otherwise each lane reads its class row
-// and out-of-box slots use the cell itself with a zero coefficient.
-template <typename TV, typename TC, int S, int NI>
-__device__ __forceinline__ TV stencil_row(const OpView<TC> &op, const TV *__restrict__ x, int64_t c, int xi, int yi, int zi) {
-    const bool fast = in_fast(xi, op.nx) && in_fast(yi, op.ny) && in_fast(zi, op.nz);
-    TV acc = TV(0);
-    if(op.table) {
-        const int z0 = __builtin_amdgcn_readfirstlane(zi);
-        if(__all(fast && zi == z0)) {
-            const TC *row = op.table_int + z0 * NI;
-#pragma unroll
-            for(int k = 0; k < NI; k++) acc += (TV)row[k] * x[c + op.lin_int[k]];
-            return acc;
-        }
-        const TC *row = op.table + ((zi * 3 + pos_cls(yi, op.ny)) * 3 + pos_cls(xi, op.nx)) * S;
-#pragma unroll
-        for(int k = 0; k < S; k++) {
-            const int xx = xi + op.off[k][0], yy = yi + op.off[k][1], zz = zi + op.off[k][2];
-            const bool in = xx >= 0 && yy >= 0 && zz >= 0 && xx < op.nx && yy < op.ny && zz < op.nz;
-            const TC a = in ? row[k] : TC(0);
-            acc += (TV)a * x[in ? c + op.lin[k] : c];
-        }
-        return acc;
-    }
-    const TC *col = op.percell + c;
-    const int64_t n = op.n;
-    if(__all(fast)) {
-#pragma unroll
-        for(int k = 0; k < S; k++) acc += (TV)col[k * n] * x[c + op.lin[k]];
-        return acc;
-    }
-#pragma unroll
-    for(int k = 0; k < S; k++) {
-        const int xx = xi + op.off[k][0], yy = yi + op.off[k][1], zz = zi + op.off[k][2];
-        const bool in = xx >= 0 && yy >= 0 && zz >= 0 && xx < op.nx && yy < op.ny && zz < op.nz;
-        const TC a = in ? col[k * n] : TC(0);
-        acc += (TV)a * x[in ? c + op.lin[k] : c];
-    }
-    return acc;
+    return !done;
 }
 
 template <typename TC> __device__ __forceinline__ double diag_of(const OpView<TC> &op, int64_t c) {
@@ -199,33 +189,281 @@ template <typename TC> __device__ __forceinline__ double diag_of(const OpView<TC
         const int xi = (int)(c % op.nx);
         const int64_t t = c / op.nx;
         const int yi = (int)(t % op.ny), zi = (int)(t / op.ny);
-        return (double)op.table[((zi * 3 + pos_cls(yi, op.ny)) * 3 + pos_cls(xi, op.nx)) * op.nslots];
+        return (double)op.table[((zi * 3 + pos_cls(yi, op.ny)) * 3 + pos_cls(xi, op.nx)) * 27 + 13];
     }
     return (double)op.percell[c];
 }
 
-__device__ __forceinline__ bool is_active(const uint8_t *a, int64_t c) { return !a || a[c]; }
 
 // ---------------------------------------------------------------------------
-// FP64 CG (plain path): x is the FP64 potential.
+// Tiled stencil sweep.  A block owns a 64 x 8 tile of an xy plane and marches
+// through z planes, keeping planes z-1, z, z+1 of the tile (plus a one-cell
+// halo, zero outside the box) in LDS; the next plane's global loads are issued
+// before the current plane is computed.  Rows follow compile-time patterns, so
+// every LDS read uses an immediate offset.
+//   per-cell mode: tiles cover the box; every cell reads its own row
+//     (pattern PAT) and the zero halo stands in for missing neighbours;
+//   class mode: tiles cover the cells whose x and y lie in the axes' interior
+//     ranges, so a plane's row is the same for every lane: the layer's
+//     interior row (pattern PAT, scalar registers) or, on the two boundary
+//     planes, the 27-slot class row.  The remaining cells (the x and y faces
+//     of the box, about 4/n of an n x n plane) are evaluated by "edge" blocks
+//     of the same launch from global memory.
+// Work items are (xy tile, z plane) pairs in tile-major order; each tile block
+// takes one contiguous run of `per` items (a few tiles' z ranges at most), and
+// the grid is one wave of resident blocks, so all blocks finish together.
+// With FUSE the value staged for the stencil is the new search direction
+// p = rz + beta * p_in (written for the block's own cells to p_out, a buffer
+// distinct from p_in, so halo reads never see updated values).
 // ---------------------------------------------------------------------------
-template <int S, int NI>
-__global__ __launch_bounds__(kBlock) void k_res64(OpView<double> op, const double *__restrict__ x, const double *__restrict__ b,
-                                                  double *__restrict__ r, double *__restrict__ p, double *__restrict__ z,
-                                                  int jacobi, double tol, double rel_tol, int max_its, CgScalars *sc,
-                                                  double *partials, unsigned *counter) {
-    double v[3] = {0.0, 0.0, 0.0};  // r.r, r.z, b.b
-    Walk w;
-    if(walk_begin(op.nx, op.ny, op.nz, w)) do {
-        const int64_t c = w.x + (int64_t)op.nx * (w.y + (int64_t)op.ny * w.z);
-        const double ax = stencil_row<double, double, S, NI>(op, x, c, w.x, w.y, w.z);
-        if(!is_active(op.active, c)) continue;
+constexpr int kTX = 64, kTY = 8, kW = kTX + 2, kH = kTY + 2, kPL = kW * kH;
+constexpr int kNLD = (kPL + kBlock - 1) / kBlock;
+constexpr int kRPT = kTX * kTY / kBlock;  // rows per thread
+
+struct TileGeom {
+    int ox, oy, lx, ly;         // tiled cells: x in [ox, ox+lx), y in [oy, oy+ly)
+    int ntx, nty;
+    int nedge_blocks, nblocks;  // edge blocks first, then tile blocks
+    int64_t nitems, per;        // (tile, plane) items and items per tile block
+    int zlo, zhi;               // planes whose tiled cells use the interior row
+    int cxf, cyf;               // (x, y) class of the tiled cells
+    // edge cells, per plane: rows y < oy and y >= oy+ly (all x), then for
+    // y in [oy, oy+ly) the cells x < ox and x >= ox+lx
+    int ring, nly, nhy, nlx, nhx;
+    int64_t nedge, eper;
+};
+
+template <typename TV, bool FUSE>
+__device__ __forceinline__ TV staged(const TV *__restrict__ x, const TV *__restrict__ rz, TV beta, int64_t c) {
+    if constexpr(FUSE) return rz[c] + beta * x[c];
+    else return x[c];
+}
+
+template <typename TC> using scalar_ptr = const __attribute__((address_space(4))) TC *;
+
+// Wave-uniform rows that no kernel writes are read through the constant
+// address space, so they land in scalar registers.
+template <typename TC> __device__ __forceinline__ scalar_ptr<TC> as_scalar(const TC *p) { return (scalar_ptr<TC>)p; }
+
+template <int P, typename TC, typename TV>
+__device__ __forceinline__ TV row_lds(const TC *row, const TV *const (&P3)[3], int base) {
+    TV acc = TV(0);
+#pragma unroll
+    for(int k = 0; k < P; k++)
+        acc += (TV)row[k] * P3[RowPattern<P>::o[k][2] + 1][base + RowPattern<P>::o[k][1] * kW + RowPattern<P>::o[k][0]];
+    return acc;
+}
+
+// Uniform 27-slot row against the LDS planes (boundary planes only); rolled
+// over the nine (oz, oy) lines to keep the register count of the kernel low.
+template <typename TC, typename TV>
+__device__ __forceinline__ TV row27_lds(scalar_ptr<TC> row, const TV *const (&P3)[3], int base) {
+    TV acc = TV(0);
+#pragma unroll 1
+    for(int l = 0; l < 9; l++) {
+        const TV *P = P3[l / 3] + base + (l % 3 - 1) * kW;
+        acc += (TV)row[3 * l] * P[-1];
+        acc += (TV)row[3 * l + 1] * P[0];
+        acc += (TV)row[3 * l + 2] * P[1];
+    }
+    return acc;
+}
+
+// Planes z0..z1-1 of xy tile t.
+template <typename TV, typename TC, bool CLS, int PAT, bool FUSE, class Epi>
+__device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeom &g, int t, int z0, int z1, const TV *__restrict__ x,
+                                             const TV *__restrict__ rz, TV beta, TV *__restrict__ p_out, const Epi &epi,
+                                             double (&v)[Epi::NR]) {
+    __shared__ TV pl[3 * kPL];
+    const int x0 = g.ox + (t % g.ntx) * kTX, y0 = g.oy + (t / g.ntx) * kTY;
+    const int tid = threadIdx.x;
+    const int64_t sz = (int64_t)op.nx * op.ny;
+    int goff[kNLD];
+    bool gok[kNLD];
+#pragma unroll
+    for(int j = 0; j < kNLD; j++) {
+        const int idx = tid + j * kBlock;
+        const int lx = idx % kW, ly = idx / kW;
+        const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+        gok[j] = idx < kPL && gx >= 0 && gx < op.nx && gy >= 0 && gy < op.ny;
+        goff[j] = gok[j] ? gx + op.nx * gy : 0;
+    }
+    TV reg[kNLD];
+    auto fetch = [&](int z) {
+        const bool zin = z >= 0 && z < op.nz;
+        const int64_t zo = sz * (zin ? z : 0);
+#pragma unroll
+        for(int j = 0; j < kNLD; j++) {
+            const TV val = staged<TV, FUSE>(x, rz, beta, zo + goff[j]);  // always a valid address
+            reg[j] = (zin && gok[j]) ? val : TV(0);
+        }
+    };
+    auto put = [&](int slot) {
+#pragma unroll
+        for(int j = 0; j < kNLD; j++)
+            if(tid + j * kBlock < kPL) pl[slot * kPL + tid + j * kBlock] = reg[j];
+    };
+    fetch(z0 - 1);
+    put((z0 + 2) % 3);
+    fetch(z0);
+    put(z0 % 3);
+    fetch(z0 + 1);
+    const int tx = tid % kTX, ty = tid / kTX;
+    const int gx = x0 + tx;
+    const bool xin = gx < g.ox + g.lx;
+    for(int z = z0; z < z1; z++) {
+        put((z + 1) % 3);
+        if(z + 1 < z1) fetch(z + 2);
+        __syncthreads();
+        const TV *const P3[3] = {pl + ((z + 2) % 3) * kPL, pl + (z % 3) * kPL, pl + ((z + 1) % 3) * kPL};
+        TC crow[PAT];
+        bool zf = true;
+        scalar_ptr<TC> brow = nullptr;
+        if constexpr(CLS) {
+            zf = z >= g.zlo && z <= g.zhi;
+            if(zf) {
+                const scalar_ptr<TC> row = as_scalar(op.table_int + z * PAT);
+#pragma unroll
+                for(int k = 0; k < PAT; k++) crow[k] = row[k];
+            } else {
+                brow = as_scalar(op.table + ((z * 3 + g.cyf) * 3 + g.cxf) * 27);
+            }
+        }
+#pragma unroll
+        for(int q = 0; q < kRPT; q++) {
+            const int rr = ty + q * (kBlock / kTX);
+            const int gy = y0 + rr;
+            if(!xin || gy >= g.oy + g.ly) continue;
+            const int64_t c = gx + (int64_t)op.nx * gy + sz * z;
+            const int base = (rr + 1) * kW + tx + 1;
+            TV acc;
+            double dg = 0.0;  // the row's diagonal (Jacobi epilogues only)
+            if constexpr(!CLS) {
+                const TC *col = op.percell + c;
+                acc = TV(0);
+#pragma unroll
+                for(int k = 0; k < PAT; k++)
+                    acc += (TV)col[(int64_t)k * op.n] *
+                           P3[RowPattern<PAT>::o[k][2] + 1][base + RowPattern<PAT>::o[k][1] * kW + RowPattern<PAT>::o[k][0]];
+                if constexpr(Epi::kDiag) dg = (double)col[0];
+            } else if(zf) {
+                acc = row_lds<PAT>(crow, P3, base);
+                if constexpr(Epi::kDiag) dg = (double)crow[0];
+            } else {
+                acc = row27_lds<TC>(brow, P3, base);
+                if constexpr(Epi::kDiag) dg = (double)brow[13];
+            }
+            const TV xc = P3[1][base];
+            if constexpr(FUSE) p_out[c] = xc;
+            epi(c, acc, xc, dg, v);
+        }
+        __syncthreads();
+    }
+}
+
+template <typename TV, typename TC, bool CLS, int PAT, bool FUSE, class Epi>
+__device__ __forceinline__ void tile_sweep(const OpView<TC> &op, const TileGeom &g, int b, int nb, const TV *__restrict__ x,
+                                           const TV *__restrict__ rz, TV beta, TV *__restrict__ p_out, const Epi &epi,
+                                           double (&v)[Epi::NR]) {
+    // runs of one XCD's blocks are adjacent (shared halos stay in its L2)
+    const int L = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    int64_t w = (int64_t)L * g.per;
+    const int64_t end = w + g.per < g.nitems ? w + g.per : g.nitems;
+    while(w < end) {
+        const int t = (int)(w / op.nz), z0 = (int)(w % op.nz);
+        const int z1 = (int64_t)op.nz - z0 < end - w ? op.nz : z0 + (int)(end - w);
+        tile_segment<TV, TC, CLS, PAT, FUSE, Epi>(op, g, t, z0, z1, x, rz, beta, p_out, epi, v);
+        w += z1 - z0;
+    }
+}
+
+// Edge cells (class mode): class rows and neighbour values from global memory.
+template <typename TV, typename TC, bool FUSE, class Epi>
+__device__ __forceinline__ void edge_sweep(const OpView<TC> &op, const TileGeom &g, int b, const TV *__restrict__ x,
+                                           const TV *__restrict__ rz, TV beta, TV *__restrict__ p_out, const Epi &epi,
+                                           double (&v)[Epi::NR]) {
+    const int64_t beg = (int64_t)b * g.eper, end = beg + g.eper < g.nedge ? beg + g.eper : g.nedge;
+    const int64_t sz = (int64_t)op.nx * op.ny;
+    const int wlo = g.nly * op.nx, whi = (g.nly + g.nhy) * op.nx, side = g.nlx + g.nhx;
+    for(int64_t i = beg + threadIdx.x; i < end; i += kBlock) {
+        const int zi = (int)(i / g.ring), r = (int)(i % g.ring);
+        int xi, yi;
+        if(r < wlo) {
+            yi = r / op.nx;
+            xi = r % op.nx;
+        } else if(r < whi) {
+            yi = g.oy + g.ly + (r - wlo) / op.nx;
+            xi = (r - wlo) % op.nx;
+        } else {
+            const int j = (r - whi) % side;
+            yi = g.oy + (r - whi) / side;
+            xi = j < g.nlx ? j : g.ox + g.lx + (j - g.nlx);
+        }
+        const int64_t c = xi + (int64_t)op.nx * yi + sz * zi;
+        const TC *row = op.table + ((zi * 3 + pos_cls(yi, op.ny)) * 3 + pos_cls(xi, op.nx)) * 27;
+        TV acc = TV(0);
+#pragma unroll 1
+        for(int l = 0; l < 9; l++) {
+            const int zz = zi + l / 3 - 1, yy = yi + l % 3 - 1;
+            const bool ok = zz >= 0 && zz < op.nz && yy >= 0 && yy < op.ny;
+            const int64_t cl = c + (int64_t)(l / 3 - 1) * sz + (int64_t)(l % 3 - 1) * op.nx;
+#pragma unroll
+            for(int ox = -1; ox <= 1; ox++) {
+                const bool in = ok && xi + ox >= 0 && xi + ox < op.nx;
+                const TV val = staged<TV, FUSE>(x, rz, beta, in ? cl + ox : c);
+                acc += (TV)row[3 * l + ox + 1] * (in ? val : TV(0));
+            }
+        }
+        const TV xc = staged<TV, FUSE>(x, rz, beta, c);
+        if constexpr(FUSE) p_out[c] = xc;
+        epi(c, acc, xc, (double)row[13], v);
+    }
+}
+
+// Residual kernels: no prologue.  Iteration kernel (q = A p): prologue first.
+template <typename TV, typename TC, bool CLS, int PAT, bool FUSE, class Epi>
+__global__ __launch_bounds__(kBlock) void k_stencil(OpView<TC> op, TileGeom g, const TV *__restrict__ x, const TV *__restrict__ rz,
+                                                    TV *__restrict__ p_out, Epi epi, CgSetup cs, int mode, int k, const double *Pin,
+                                                    int nprev, double *Pout, CgScalars *sc, CgScalars *so) {
+    double betad = 0.0;
+    if(mode >= 0 && !iter_prologue(mode, k, Pin, nprev, sc, so, cs, betad)) return;
+    const TV beta = (TV)betad;
+    double v[Epi::NR];
+#pragma unroll
+    for(int j = 0; j < Epi::NR; j++) v[j] = 0.0;
+    const int b = blockIdx.x;
+    if constexpr(CLS) {
+        if(b < g.nedge_blocks) edge_sweep<TV, TC, FUSE, Epi>(op, g, b, x, rz, beta, p_out, epi, v);
+        else tile_sweep<TV, TC, CLS, PAT, FUSE, Epi>(op, g, b - g.nedge_blocks, g.nblocks - g.nedge_blocks, x, rz, beta, p_out, epi, v);
+    } else {
+        tile_sweep<TV, TC, CLS, PAT, FUSE, Epi>(op, g, b, g.nblocks, x, rz, beta, p_out, epi, v);
+    }
+    write_partials<Epi::NR>(v, Pout);
+}
+
+// q = A p, partial p.q
+template <typename TV> struct EpiSpmv {
+    static constexpr int NR = 1;
+    static constexpr bool kDiag = false;
+    TV *q;
+    __device__ void operator()(int64_t c, TV ax, TV pc, double, double (&v)[NR]) const {
+        q[c] = ax;
+        v[0] += (double)pc * (double)ax;
+    }
+};
+
+// r = b - A x, p = r (or z = D^-1 r, p = z); partial r.r, r.z, b.b
+template <bool JACOBI> struct EpiRes64 {
+    static constexpr int NR = 3;
+    static constexpr bool kDiag = JACOBI;
+    const double *b;
+    double *r, *p, *z;
+    __device__ void operator()(int64_t c, double ax, double, double dg, double (&v)[NR]) const {
         const double bc = b[c];
         const double rc = bc - ax;
         r[c] = rc;
-        if(jacobi) {
-            double d = diag_of(op, c);
-            if(d == 0.0) d = 1.0;
+        if constexpr(JACOBI) {
+            const double d = dg == 0.0 ? 1.0 : dg;
             const double zc = (1.0 / d) * rc;
             z[c] = zc;
             p[c] = zc;
@@ -235,52 +473,48 @@ __global__ __launch_bounds__(kBlock) void k_res64(OpView<double> op, const doubl
         }
         v[0] += rc * rc;
         v[2] += bc * bc;
-    } while(walk_next(op.nx, w));
-    if(grid_sum<3>(v, partials, counter) && threadIdx.x == 0) {
-        const double tol_abs = rel_tol > 0.0 ? rel_tol * rel_tol * v[2] : tol;
-        sc->rr_true = v[0];
-        sc->rr = jacobi ? v[1] : v[0];
-        sc->rr0 = v[0];
-        sc->tol_abs = tol_abs;
-        sc->iters = 1;
-        sc->max_its = max_its;
-        sc->done = !(v[0] >= tol_abs) || max_its <= 1;
     }
-}
+};
 
-template <typename TV, typename TC, int S, int NI>
-__global__ __launch_bounds__(kBlock) void k_spmv_dot(OpView<TC> op, const TV *__restrict__ p, TV *__restrict__ q, CgScalars *sc,
-                                                     double *partials, unsigned *counter) {
-    if(sc->done) return;
-    double v[1] = {0.0};
-    Walk w;
-    if(walk_begin(op.nx, op.ny, op.nz, w)) do {
-        const int64_t c = w.x + (int64_t)op.nx * (w.y + (int64_t)op.ny * w.z);
-        const TV qc = stencil_row<TV, TC, S, NI>(op, p, c, w.x, w.y, w.z);
-        if(!is_active(op.active, c)) continue;
-        q[c] = qc;
-        v[0] += (double)p[c] * (double)qc;
-    } while(walk_next(op.nx, w));
-    if(grid_sum<1>(v, partials, counter) && threadIdx.x == 0) {
-        sc->pAp = v[0];
-        sc->alpha = sc->rr / v[0];
+// mixed-precision outer residual r = b - A x in FP64, handed to the FP32 solve
+struct EpiResOuter {
+    static constexpr int NR = 2;  // r.r, b.b
+    static constexpr bool kDiag = false;
+    const double *b;
+    float *r32, *p32, *d32;
+    __device__ void operator()(int64_t c, double ax, double, double, double (&v)[NR]) const {
+        const double bc = b[c];
+        const double rc = bc - ax;
+        const float rf = (float)rc;
+        r32[c] = rf;
+        p32[c] = rf;
+        d32[c] = 0.0f;
+        v[0] += rc * rc;
+        v[1] += bc * bc;
     }
-}
+};
 
-// Elementwise kernels walk contiguous chunks with four cells in flight per
-// thread (loads of the four are issued before any use).
+// Elementwise kernels keep four cells in flight per thread (all loads issued
+// before any use).
 constexpr int kIlp = 4;
 
+// x += alpha p, r -= alpha q (z = D^-1 r); partial r.r, r.z.  Prologue: p.q.
 template <typename TV, typename TX, typename TC>
 __global__ __launch_bounds__(kBlock) void k_update(OpView<TC> op, TX *__restrict__ x, const TV *__restrict__ p, const TV *__restrict__ q,
-                                                   TV *__restrict__ r, TV *__restrict__ z, int jacobi, CgScalars *sc,
-                                                   double *partials, unsigned *counter) {
+                                                   TV *__restrict__ r, TV *__restrict__ z, int jacobi, int k, const double *Pin,
+                                                   int nprev, double *Pout, CgScalars *sc) {
     if(sc->done) return;
-    const double alpha = sc->alpha;
+    double pq[1];
+    sum_partials<1>(Pin, nprev, pq);
+    const double alpha = sc->rrs[k & 1] / pq[0];
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->pAp = pq[0];
+        sc->alpha = alpha;
+    }
     const TV a = (TV)alpha;
     int64_t beg, end;
     chunk_range(op.n, beg, end);
-    double v[2] = {0.0, 0.0};  // r.r, r.z
+    double v[2] = {0.0, 0.0};
     for(int64_t c0 = beg + threadIdx.x; c0 < end; c0 += kIlp * kBlock) {
         int64_t cc[kIlp];
         TX xv[kIlp];
@@ -297,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void k_update(OpView<TC> op, TX *__restrict
 #pragma unroll
         for(int u = 0; u < kIlp; u++) {
             const int64_t c = c0 + u * kBlock;
-            if(c >= end || !is_active(op.active, c)) continue;
+            if(c >= end) continue;
             x[c] = xv[u] + (TX)(a * pv[u]);
             const TV rc = rv[u] - a * qv[u];
             r[c] = rc;
@@ -311,91 +545,11 @@ __global__ __launch_bounds__(kBlock) void k_update(OpView<TC> op, TX *__restrict
             v[0] += (double)rc * (double)rc;
         }
     }
-    if(grid_sum<2>(v, partials, counter) && threadIdx.x == 0) {
-        const double rz_new = jacobi ? v[1] : v[0];
-        sc->beta = rz_new / sc->rr;
-        sc->rr = rz_new;
-        sc->rr_true = v[0];
-        sc->iters = sc->iters + 1;
-        sc->done = (v[0] <= sc->tol_abs) || (sc->iters >= sc->max_its) || (v[0] != v[0]);
-    }
+    write_partials<2>(v, Pout);
 }
 
-template <typename TV, typename TC>
-__global__ __launch_bounds__(kBlock) void k_dir(OpView<TC> op, TV *__restrict__ p, const TV *__restrict__ rz, CgScalars *sc) {
-    if(sc->done) return;
-    const TV beta = (TV)sc->beta;
-    int64_t beg, end;
-    chunk_range(op.n, beg, end);
-    for(int64_t c0 = beg + threadIdx.x; c0 < end; c0 += kIlp * kBlock) {
-        int64_t cc[kIlp];
-        TV pv[kIlp], rv[kIlp];
-#pragma unroll
-        for(int u = 0; u < kIlp; u++) {
-            const int64_t c = c0 + u * kBlock;
-            cc[u] = c < end ? c : end - 1;
-            pv[u] = p[cc[u]];
-            rv[u] = rz[cc[u]];
-        }
-#pragma unroll
-        for(int u = 0; u < kIlp; u++) {
-            const int64_t c = c0 + u * kBlock;
-            if(c >= end || !is_active(op.active, c)) continue;
-            p[c] = rv[u] + beta * pv[u];
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Mixed precision: outer residual in FP64, FP32 inner CG, FP64 correction.
-// ---------------------------------------------------------------------------
-template <int S, int NI>
-__global__ __launch_bounds__(kBlock) void k_res_outer(OpView<double> op, const double *__restrict__ x, const double *__restrict__ b,
-                                                      float *__restrict__ r32, float *__restrict__ p32, float *__restrict__ d32,
-                                                      int first, double tol, double rel_tol, int max_outer, double inner_rel,
-                                                      int max_inner, CgScalars *so, CgScalars *si, double *partials,
-                                                      unsigned *counter) {
-    if(!first && so->done) return;
-    double v[2] = {0.0, 0.0};  // r.r, b.b
-    Walk w;
-    if(walk_begin(op.nx, op.ny, op.nz, w)) do {
-        const int64_t c = w.x + (int64_t)op.nx * (w.y + (int64_t)op.ny * w.z);
-        const double ax = stencil_row<double, double, S, NI>(op, x, c, w.x, w.y, w.z);
-        if(!is_active(op.active, c)) continue;
-        const double bc = b[c];
-        const double rc = bc - ax;
-        const float rf = (float)rc;
-        r32[c] = rf;
-        p32[c] = rf;
-        d32[c] = 0.0f;
-        v[0] += rc * rc;
-        v[1] += bc * bc;
-    } while(walk_next(op.nx, w));
-    if(grid_sum<2>(v, partials, counter) && threadIdx.x == 0) {
-        if(first) {
-            so->tol_abs = rel_tol > 0.0 ? rel_tol * rel_tol * v[1] : tol;
-            so->iters = 0;
-            so->max_its = max_outer;
-            so->rr0 = v[0];
-        }
-        so->rr_true = v[0];
-        so->rr = v[0];
-        so->done = !(v[0] > so->tol_abs) || so->iters >= so->max_its;
-        so->iters = so->iters + 1;
-        si->rr = v[0];
-        si->rr_true = v[0];
-        si->rr0 = v[0];
-        // inner target: relative reduction, but no tighter than half the outer target
-        const double t_rel = inner_rel * inner_rel * v[0];
-        si->tol_abs = t_rel > 0.5 * so->tol_abs ? t_rel : 0.5 * so->tol_abs;
-        si->iters = 0;
-        si->max_its = max_inner;
-        si->done = so->done;
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_correct(int64_t n, const uint8_t *active, double *__restrict__ x,
-                                                    const float *__restrict__ d32, const CgScalars *so) {
+__global__ __launch_bounds__(kBlock) void k_correct(int64_t n, double *__restrict__ x, const float *__restrict__ d32,
+                                                    const CgScalars *so) {
     if(so->done) return;
     int64_t beg, end;
     chunk_range(n, beg, end);
@@ -413,8 +567,7 @@ __global__ __launch_bounds__(kBlock) void k_correct(int64_t n, const uint8_t *ac
 #pragma unroll
         for(int u = 0; u < kIlp; u++) {
             const int64_t c = c0 + u * kBlock;
-            if(c >= end || !is_active(active, c)) continue;
-            x[c] = xv[u] + (double)dv[u];
+            if(c < end) x[c] = xv[u] + (double)dv[u];
         }
     }
 }
@@ -424,24 +577,20 @@ __global__ __launch_bounds__(kBlock) void k_correct(int64_t n, const uint8_t *ac
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_csr_spmv(int n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
                                                      const double *__restrict__ val, const double *__restrict__ x,
-                                                     double *__restrict__ y) {
+                                                     double *__restrict__ y, const CgScalars *sc) {
+    if(sc && sc->done) return;
     const int lane = threadIdx.x & 63;
     const int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
     if(row >= n) return;
     double acc = 0.0;
-    for(int64_t k = rp[row] + lane; k < rp[row + 1]; k += 64) acc += val[k] * x[col[k]];
+    for(int64_t kk = rp[row] + lane; kk < rp[row + 1]; kk += 64) acc += val[kk] * x[col[kk]];
     acc = wave_sum(acc);
     if(lane == 0) y[row] = acc;
 }
 
-// dot products and vector updates of the CSR solve reuse the stencil kernels
-// through an operator view with no off-diagonal slots (only the active mask
-// and the diagonal for Jacobi are consulted).
-
 __global__ __launch_bounds__(kBlock) void k_csr_res(int n, const double *__restrict__ Ax, const double *__restrict__ b,
                                                     const double *__restrict__ diag, double *__restrict__ r, double *__restrict__ p,
-                                                    double *__restrict__ z, int jacobi, double tol, double rel_tol, int max_its,
-                                                    CgScalars *sc, double *partials, unsigned *counter) {
+                                                    double *__restrict__ z, int jacobi, double *Pout) {
     int64_t beg, end;
     chunk_range(n, beg, end);
     double v[3] = {0.0, 0.0, 0.0};
@@ -461,37 +610,41 @@ __global__ __launch_bounds__(kBlock) void k_csr_res(int n, const double *__restr
         v[0] += rc * rc;
         v[2] += b[c] * b[c];
     }
-    if(grid_sum<3>(v, partials, counter) && threadIdx.x == 0) {
-        const double tol_abs = rel_tol > 0.0 ? rel_tol * rel_tol * v[2] : tol;
-        sc->rr_true = v[0];
-        sc->rr = jacobi ? v[1] : v[0];
-        sc->rr0 = v[0];
-        sc->tol_abs = tol_abs;
-        sc->iters = 1;
-        sc->max_its = max_its;
-        sc->done = !(v[0] >= tol_abs) || max_its <= 1;
-    }
+    write_partials<3>(v, Pout);
 }
 
-__global__ __launch_bounds__(kBlock) void k_dot_pq(int n, const double *__restrict__ p, const double *__restrict__ q, CgScalars *sc,
-                                                   double *partials, unsigned *counter) {
+// p = rz + beta p after the stopping test (prologue over the previous partials)
+__global__ __launch_bounds__(kBlock) void k_csr_dir(int n, double *__restrict__ p, const double *__restrict__ rz, CgSetup cs, int mode,
+                                                    int k, const double *Pin, int nprev, CgScalars *sc) {
+    double beta = 0.0;
+    if(!iter_prologue(mode, k, Pin, nprev, sc, nullptr, cs, beta)) return;
+    int64_t beg, end;
+    chunk_range(n, beg, end);
+    for(int64_t c = beg + threadIdx.x; c < end; c += kBlock) p[c] = rz[c] + beta * p[c];
+}
+
+__global__ __launch_bounds__(kBlock) void k_csr_dot(int n, const double *__restrict__ p, const double *__restrict__ q, double *Pout,
+                                                    const CgScalars *sc) {
     if(sc->done) return;
     int64_t beg, end;
     chunk_range(n, beg, end);
     double v[1] = {0.0};
     for(int64_t c = beg + threadIdx.x; c < end; c += kBlock) v[0] += p[c] * q[c];
-    if(grid_sum<1>(v, partials, counter) && threadIdx.x == 0) {
-        sc->pAp = v[0];
-        sc->alpha = sc->rr / v[0];
-    }
+    write_partials<1>(v, Pout);
 }
 
 __global__ __launch_bounds__(kBlock) void k_csr_update(int n, double *__restrict__ x, const double *__restrict__ p,
                                                        const double *__restrict__ q, double *__restrict__ r, double *__restrict__ z,
-                                                       const double *__restrict__ diag, int jacobi, CgScalars *sc,
-                                                       double *partials, unsigned *counter) {
+                                                       const double *__restrict__ diag, int jacobi, int k, const double *Pin, int nprev,
+                                                       double *Pout, CgScalars *sc) {
     if(sc->done) return;
-    const double a = sc->alpha;
+    double pq[1];
+    sum_partials<1>(Pin, nprev, pq);
+    const double a = sc->rrs[k & 1] / pq[0];
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->pAp = pq[0];
+        sc->alpha = a;
+    }
     int64_t beg, end;
     chunk_range(n, beg, end);
     double v[2] = {0.0, 0.0};
@@ -508,22 +661,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_update(int n, double *__restrict
         }
         v[0] += rc * rc;
     }
-    if(grid_sum<2>(v, partials, counter) && threadIdx.x == 0) {
-        const double rz_new = jacobi ? v[1] : v[0];
-        sc->beta = rz_new / sc->rr;
-        sc->rr = rz_new;
-        sc->rr_true = v[0];
-        sc->iters = sc->iters + 1;
-        sc->done = (v[0] <= sc->tol_abs) || (sc->iters >= sc->max_its) || (v[0] != v[0]);
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_csr_dir(int n, double *__restrict__ p, const double *__restrict__ rz, CgScalars *sc) {
-    if(sc->done) return;
-    const double beta = sc->beta;
-    int64_t beg, end;
-    chunk_range(n, beg, end);
-    for(int64_t c = beg + threadIdx.x; c < end; c += kBlock) p[c] = rz[c] + beta * p[c];
+    write_partials<2>(v, Pout);
 }
 
 // ---------------------------------------------------------------------------
@@ -531,48 +669,126 @@ __global__ __launch_bounds__(kBlock) void k_csr_dir(int n, double *__restrict__ 
 // ---------------------------------------------------------------------------
 template <int A> using IC = std::integral_constant<int, A>;
 
-// Call f(IC<S>, IC<NI>) for the operator's padded slot counts.
-template <class F> static void with_slots(int S, int NI, F &&f) {
-    if(S == 7) f(IC<7>{}, IC<7>{});
-    else if(S == 19) {
-        if(NI == 19) f(IC<19>{}, IC<19>{});
-        else f(IC<19>{}, IC<7>{});
-    } else {
-        if(NI == 27) f(IC<27>{}, IC<27>{});
-        else if(NI == 19) f(IC<27>{}, IC<19>{});
-        else f(IC<27>{}, IC<7>{});
-    }
+template <class F> static void with_pat(int P, F &&f) {
+    if(P == 7) f(IC<7>{});
+    else if(P == 19) f(IC<19>{});
+    else f(IC<27>{});
 }
+
+// Call f(IC<S>, IC<PAT>) for the operator's class-row slot count and row pattern.
+// Call f(IC<CLS>, IC<PAT>) for the operator's storage mode and row pattern.
+template <typename TC, class F> static void with_op(const OpView<TC> &op, F &&f) {
+    with_pat(op.pat, [&](auto P_) {
+        if(op.table) f(IC<1>{}, P_);
+        else f(IC<0>{}, P_);
+    });
+}
+
+// Resident blocks of a kernel on the current device (one full wave of blocks).
+static int resident_blocks(const void *kernel) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if(hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+       hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || cus <= 0 || per_cu <= 0)
+        return 1024;
+    return cus * per_cu;
+}
+
+// Tiles over the interior xy range (class mode) or the whole box (per-cell
+// mode); (tile, plane) items split over one wave of resident blocks with at
+// least kMinRun planes per block; edge blocks get a share of the wave in
+// proportion to their cells (an edge cell costs about kEdgeCost tiled cells).
+constexpr int kMinRun = 8, kEdgeCost = 6;
+template <typename TC> static TileGeom make_geom(const OpView<TC> &op, int slots) {
+    TileGeom g{};
+    const bool cls = op.table != nullptr;
+    int flo[3], fhi[3];
+    const int L[3] = {op.nx, op.ny, op.nz};
+    for(int a = 0; a < 3; a++) {
+        // interior range of an axis; extent 1: its single cell; extent 2: empty
+        flo[a] = L[a] == 1 ? 0 : 1;
+        fhi[a] = L[a] == 1 ? 0 : L[a] - 2;
+    }
+    g.zlo = flo[2];
+    g.zhi = fhi[2];
+    if(cls) {
+        g.ox = flo[0];
+        g.oy = flo[1];
+        g.lx = fhi[0] - flo[0] + 1 > 0 ? fhi[0] - flo[0] + 1 : 0;
+        g.ly = fhi[1] - flo[1] + 1 > 0 ? fhi[1] - flo[1] + 1 : 0;
+        g.cxf = g.lx ? pos_cls(g.ox, op.nx) : 1;
+        g.cyf = g.ly ? pos_cls(g.oy, op.ny) : 1;
+        g.nlx = g.ox;
+        g.nhx = op.nx - g.ox - g.lx;
+        g.nly = g.oy;
+        g.nhy = op.ny - g.oy - g.ly;
+        g.ring = (g.nly + g.nhy) * op.nx + g.ly * (g.nlx + g.nhx);
+        g.nedge = (int64_t)g.ring * op.nz;
+    } else {
+        g.lx = op.nx;
+        g.ly = op.ny;
+    }
+    g.ntx = (g.lx + kTX - 1) / kTX;
+    g.nty = (g.ly + kTY - 1) / kTY;
+    g.nitems = g.lx && g.ly ? (int64_t)g.ntx * g.nty * op.nz : 0;
+    int64_t ne = 0;
+    if(g.nedge > 0) {
+        ne = (int64_t)((double)slots * kEdgeCost * (double)g.nedge / (double)(kEdgeCost * g.nedge + (int64_t)g.lx * g.ly * op.nz));
+        const int64_t cap = (g.nedge + kBlock - 1) / kBlock;
+        if(ne > cap) ne = cap;
+        if(ne < 1) ne = 1;
+        if(ne > 1024) ne = 1024;
+    }
+    int64_t nt = (g.nitems + kMinRun - 1) / kMinRun;
+    if(nt > slots - ne) nt = slots - ne;
+    if(nt > kMaxBlocks - ne) nt = kMaxBlocks - ne;
+    if(nt >= 16) nt = nt / 8 * 8;  // whole rounds of the XCD dispatch order
+    if(nt < 1) nt = g.nitems || !ne ? 1 : 0;
+    g.nedge_blocks = (int)ne;
+    g.nblocks = (int)(ne + nt);
+    g.per = nt ? (g.nitems + nt - 1) / nt : 0;
+    g.eper = ne ? (g.nedge + ne - 1) / ne : 0;
+    return g;
+}
+
+// Geometry for one kernel instance, computed once per device.
+template <auto K, typename TC> static TileGeom geom_for(const OpView<TC> &op) {
+    static int slots[64] = {};
+    int dev = 0;
+    if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if(!slots[dev]) slots[dev] = resident_blocks(reinterpret_cast<const void *>(K));
+    return make_geom(op, slots[dev]);
+}
+
+static int check_pattern(int pat) {
+    if(pat != 7 && pat != 19 && pat != 27) {
+        mono_set_error("unsupported row pattern %d", pat);
+        return -1;
+    }
+    return 0;
+}
+
 static int grid_for(int64_t n) {
     int64_t g = (n + kBlock - 1) / kBlock;
-    if(g > kMaxBlocks) g = kMaxBlocks;
-    if(g >= 64) g = g / 8 * 8;  // multiple of 8 for the XCD remap
+    if(g > kElemBlocks) g = kElemBlocks;
+    if(g >= 64) g = g / 8 * 8;  // multiple of 8 for the XCD chunk order
     return (int)(g < 1 ? 1 : g);
 }
 
 int CgWorkspace::init(int64_t n_, int64_t) {
     n = n_;
     const size_t b64 = sizeof(double) * (size_t)(n > 0 ? n : 1), b32 = sizeof(float) * (size_t)(n > 0 ? n : 1);
-    MONO_CHECK(hipMalloc(&r, b64));
-    MONO_CHECK(hipMalloc(&p, b64));
-    MONO_CHECK(hipMalloc(&q, b64));
-    MONO_CHECK(hipMalloc(&z, b64));
-    MONO_CHECK(hipMalloc(&r32, b32));
-    MONO_CHECK(hipMalloc(&p32, b32));
-    MONO_CHECK(hipMalloc(&q32, b32));
-    MONO_CHECK(hipMalloc(&d32, b32));
-    // inactive cells are never written: keep them at zero
-    MONO_CHECK(hipMemset(r, 0, b64));
-    MONO_CHECK(hipMemset(p, 0, b64));
-    MONO_CHECK(hipMemset(q, 0, b64));
-    MONO_CHECK(hipMemset(z, 0, b64));
-    MONO_CHECK(hipMemset(r32, 0, b32));
-    MONO_CHECK(hipMemset(p32, 0, b32));
-    MONO_CHECK(hipMemset(q32, 0, b32));
-    MONO_CHECK(hipMemset(d32, 0, b32));
-    MONO_CHECK(hipMalloc(&partials, sizeof(double) * kMaxBlocks * kMaxRed));
-    MONO_CHECK(hipMalloc(&counter, sizeof(unsigned)));
-    MONO_CHECK(hipMemset(counter, 0, sizeof(unsigned)));
+    double **v64[] = {&r, &p, &p2, &q, &z};
+    for(double **ptr : v64) {
+        MONO_CHECK(hipMalloc(ptr, b64));
+        MONO_CHECK(hipMemset(*ptr, 0, b64));  // inactive cells are never written
+    }
+    float **v32[] = {&r32, &p32, &p32b, &q32, &d32};
+    for(float **ptr : v32) {
+        MONO_CHECK(hipMalloc(ptr, b32));
+        MONO_CHECK(hipMemset(*ptr, 0, b32));
+    }
+    MONO_CHECK(hipMalloc(&P0, sizeof(double) * kMaxBlocks * kMaxRed));
+    MONO_CHECK(hipMalloc(&P1, sizeof(double) * kMaxBlocks * kMaxRed));
     MONO_CHECK(hipMalloc(&sc, sizeof(CgScalars)));
     MONO_CHECK(hipMalloc(&sc_in, sizeof(CgScalars)));
     MONO_CHECK(hipMemset(sc, 0, sizeof(CgScalars)));
@@ -582,19 +798,17 @@ int CgWorkspace::init(int64_t n_, int64_t) {
 }
 
 int CgWorkspace::release() {
-    double *d64[] = {r, p, q, z, partials};
+    double *d64[] = {r, p, p2, q, z, P0, P1};
     for(double *ptr : d64)
         if(ptr) MONO_CHECK(hipFree(ptr));
-    float *d32s[] = {r32, p32, q32, d32};
+    float *d32s[] = {r32, p32, p32b, q32, d32};
     for(float *ptr : d32s)
         if(ptr) MONO_CHECK(hipFree(ptr));
-    if(counter) MONO_CHECK(hipFree(counter));
     if(sc) MONO_CHECK(hipFree(sc));
     if(sc_in) MONO_CHECK(hipFree(sc_in));
     if(host_sc) MONO_CHECK(hipHostFree(host_sc));
-    r = p = q = z = partials = nullptr;
-    r32 = p32 = q32 = d32 = nullptr;
-    counter = nullptr;
+    r = p = p2 = q = z = P0 = P1 = nullptr;
+    r32 = p32 = p32b = q32 = d32 = nullptr;
     sc = sc_in = host_sc = nullptr;
     return 0;
 }
@@ -605,24 +819,50 @@ static int poll(CgScalars *dev, CgScalars *host, hipStream_t s) {
     return 0;
 }
 
+// Launch one stencil kernel instance over its balanced geometry; returns its
+// block count (the number of partial sums it leaves).
+template <typename TV, typename TC, int C, int P, bool FUSE, class Epi>
+static int launch_stencil(const OpView<TC> &op, const TV *x, const TV *rz, TV *p_out, const Epi &epi, const CgSetup &cs, int mode,
+                          int k, const double *Pin, int nprev, double *Pout, CgScalars *sc, CgScalars *so, hipStream_t s) {
+    constexpr auto K = &k_stencil<TV, TC, C != 0, P, FUSE, Epi>;
+    const TileGeom g = geom_for<K>(op);
+    K<<<g.nblocks, kBlock, 0, s>>>(op, g, x, rz, p_out, epi, cs, mode, k, Pin, nprev, Pout, sc, so);
+    return g.nblocks;
+}
+
 int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x, const CgConfig &cfg, CgStats *st, hipStream_t s) {
     const int grid = grid_for(op.n);
-    with_slots(op.nslots, op.nint, [&](auto S_, auto NI_) {
-        k_res64<decltype(S_)::value, decltype(NI_)::value><<<grid, kBlock, 0, s>>>(op, x, b, r, p, z, cfg.jacobi, cfg.tol, cfg.rel_tol,
-                                                                                   cfg.max_its, sc, partials, counter);
+    if(check_pattern(op.pat)) return -1;
+    const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, cfg.jacobi, cfg.max_outer, 0};
+    double *pb[2] = {p, p2};
+    // residual: r = b - A x, p = r (or z), partials r.r, r.z, b.b -> P0
+    int nres = 0;
+    with_op(op, [&](auto C_, auto P_) {
+        constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
+        if(cfg.jacobi)
+            nres = launch_stencil<double, double, C, P, false>(op, x, nullptr, nullptr, EpiRes64<true>{b, r, pb[0], z}, cs, -1, 0,
+                                                               nullptr, 0, P0, sc, nullptr, s);
+        else
+            nres = launch_stencil<double, double, C, P, false>(op, x, nullptr, nullptr, EpiRes64<false>{b, r, pb[0], z}, cs, -1, 0,
+                                                               nullptr, 0, P0, sc, nullptr, s);
     });
     MONO_CHECK(hipGetLastError());
     const double *rz = cfg.jacobi ? z : r;
     int enq = 0;
     int chunk = cfg.poll_chunk > 0 ? cfg.poll_chunk : (last_iters > 2 ? last_iters - 1 : 2);
     for(;;) {
-        for(int k = 0; k < chunk; k++) {
-            with_slots(op.nslots, op.nint, [&](auto S_, auto NI_) {
-                k_spmv_dot<double, double, decltype(S_)::value, decltype(NI_)::value>
-                    <<<grid, kBlock, 0, s>>>(op, p, q, sc, partials, counter);
+        for(int j = 0; j < chunk; j++) {
+            const int k = enq + j;
+            double *pin = pb[k & 1], *pout = pb[(k + 1) & 1];
+            // iteration k: stopping test and beta from P0; p = rz + beta p; q = A p; p.q -> P1
+            int nst = 0;
+            with_op(op, [&](auto C_, auto P_) {
+                constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
+                nst = launch_stencil<double, double, C, P, true>(op, pin, rz, pout, EpiSpmv<double>{q}, cs, k == 0 ? 0 : 1, k, P0,
+                                                                 k == 0 ? nres : grid, P1, sc, nullptr, s);
             });
-            k_update<double, double, double><<<grid, kBlock, 0, s>>>(op, x, p, q, r, z, cfg.jacobi, sc, partials, counter);
-            k_dir<double, double><<<grid, kBlock, 0, s>>>(op, p, rz, sc);
+            // alpha from P1; x += alpha p; r -= alpha q; r.r, r.z -> P0
+            k_update<double, double, double><<<grid, kBlock, 0, s>>>(op, x, pout, q, r, z, cfg.jacobi, k, P1, nst, P0, sc);
         }
         MONO_CHECK(hipGetLastError());
         enq += chunk;
@@ -643,44 +883,47 @@ int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x
 int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op32, const double *b, double *x, const CgConfig &cfg,
                              CgStats *st, hipStream_t s) {
     const int grid = grid_for(op64.n);
+    if(check_pattern(op64.pat)) return -1;
+    float *pb[2] = {p32, p32b};
     int inner_total = 0;
     int chunk0 = last_inner > 2 ? last_inner - 1 : 4;
     CgScalars *h_out = host_sc, *h_in = host_sc + 1;
     for(int sweep = 0;; sweep++) {
-        with_slots(op64.nslots, op64.nint, [&](auto S_, auto NI_) {
-            k_res_outer<decltype(S_)::value, decltype(NI_)::value><<<grid, kBlock, 0, s>>>(
-                op64, x, b, r32, p32, d32, sweep == 0, cfg.tol, cfg.rel_tol, cfg.max_outer, cfg.inner_rel, cfg.max_its, sc, sc_in,
-                partials, counter);
+        const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, 0, cfg.max_outer, sweep};
+        // outer residual in FP64 -> r32 = p32 = r, d32 = 0; r.r, b.b -> P0
+        int nres = 0;
+        with_op(op64, [&](auto C_, auto P_) {
+            constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
+            nres = launch_stencil<double, double, C, P, false>(op64, x, nullptr, nullptr, EpiResOuter{b, r32, pb[0], d32}, cs, -1, 0,
+                                                               nullptr, 0, P0, sc, nullptr, s);
         });
         MONO_CHECK(hipGetLastError());
         int chunk = cfg.poll_chunk > 0 ? cfg.poll_chunk : chunk0;
         int enq = 0;
-        bool outer_done = false;
         for(;;) {
-            for(int k = 0; k < chunk; k++) {
-                with_slots(op32.nslots, op32.nint, [&](auto S_, auto NI_) {
-                    k_spmv_dot<float, float, decltype(S_)::value, decltype(NI_)::value>
-                        <<<grid, kBlock, 0, s>>>(op32, p32, q32, sc_in, partials, counter);
+            for(int j = 0; j < chunk; j++) {
+                const int k = enq + j;
+                float *pin = pb[k & 1], *pout = pb[(k + 1) & 1];
+                int nst = 0;
+                with_op(op32, [&](auto C_, auto P_) {
+                    constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
+                    nst = launch_stencil<float, float, C, P, true>(op32, pin, r32, pout, EpiSpmv<float>{q32}, cs, k == 0 ? 2 : 1, k, P0,
+                                                                   k == 0 ? nres : grid, P1, sc_in, sc, s);
                 });
-                k_update<float, float, float><<<grid, kBlock, 0, s>>>(op32, d32, p32, q32, r32, nullptr, 0, sc_in, partials, counter);
-                k_dir<float, float><<<grid, kBlock, 0, s>>>(op32, p32, r32, sc_in);
+                k_update<float, float, float><<<grid, kBlock, 0, s>>>(op32, d32, pout, q32, r32, nullptr, 0, k, P1, nst, P0, sc_in);
             }
             MONO_CHECK(hipGetLastError());
             enq += chunk;
             MONO_CHECK(hipMemcpyAsync(h_out, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
             MONO_CHECK(hipMemcpyAsync(h_in, sc_in, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
             MONO_CHECK(hipStreamSynchronize(s));
-            if(h_out->done) {
-                outer_done = true;
-                break;
-            }
             if(h_in->done || enq > cfg.max_its + 2) break;
             chunk = cfg.poll_chunk > 0 ? cfg.poll_chunk : 2;
         }
-        if(outer_done) break;
-        inner_total += h_in->iters;
-        if(sweep == 0) last_inner = h_in->iters;
-        k_correct<<<grid, kBlock, 0, s>>>(op64.n, op64.active, x, d32, sc);
+        if(h_out->done) break;
+        inner_total += h_in->iters - 1;
+        if(sweep == 0) last_inner = h_in->iters - 1;
+        k_correct<<<grid, kBlock, 0, s>>>(op64.n, x, d32, sc);
         MONO_CHECK(hipGetLastError());
         chunk0 = 2;
     }
@@ -696,18 +939,20 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
 int CgWorkspace::solve_csr(const CsrMatrix &A, const double *b, double *x, const CgConfig &cfg, CgStats *st, hipStream_t s) {
     const int grid = grid_for(A.n);
     const unsigned sgrid = (unsigned)(((int64_t)A.n * 64 + kBlock - 1) / kBlock);
-    k_csr_spmv<<<sgrid, kBlock, 0, s>>>(A.n, A.row_ptr, A.col, A.val, x, q);
-    k_csr_res<<<grid, kBlock, 0, s>>>(A.n, q, b, A.diag, r, p, z, cfg.jacobi, cfg.tol, cfg.rel_tol, cfg.max_its, sc, partials, counter);
+    const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, cfg.jacobi, cfg.max_outer, 0};
+    k_csr_spmv<<<sgrid, kBlock, 0, s>>>(A.n, A.row_ptr, A.col, A.val, x, q, nullptr);
+    k_csr_res<<<grid, kBlock, 0, s>>>(A.n, q, b, A.diag, r, p, z, cfg.jacobi, P0);
     MONO_CHECK(hipGetLastError());
     const double *rz = cfg.jacobi ? z : r;
     int enq = 0;
     int chunk = cfg.poll_chunk > 0 ? cfg.poll_chunk : 4;
     for(;;) {
-        for(int k = 0; k < chunk; k++) {
-            k_csr_spmv<<<sgrid, kBlock, 0, s>>>(A.n, A.row_ptr, A.col, A.val, p, q);
-            k_dot_pq<<<grid, kBlock, 0, s>>>(A.n, p, q, sc, partials, counter);
-            k_csr_update<<<grid, kBlock, 0, s>>>(A.n, x, p, q, r, z, A.diag, cfg.jacobi, sc, partials, counter);
-            k_csr_dir<<<grid, kBlock, 0, s>>>(A.n, p, rz, sc);
+        for(int j = 0; j < chunk; j++) {
+            const int k = enq + j;
+            k_csr_dir<<<grid, kBlock, 0, s>>>(A.n, p, rz, cs, k == 0 ? 0 : 1, k, P0, grid, sc);
+            k_csr_spmv<<<sgrid, kBlock, 0, s>>>(A.n, A.row_ptr, A.col, A.val, p, q, sc);
+            k_csr_dot<<<grid, kBlock, 0, s>>>(A.n, p, q, P1, sc);
+            k_csr_update<<<grid, kBlock, 0, s>>>(A.n, x, p, q, r, z, A.diag, cfg.jacobi, k, P1, grid, P0, sc);
         }
         MONO_CHECK(hipGetLastError());
         enq += chunk;

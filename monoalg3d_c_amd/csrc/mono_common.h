@@ -31,10 +31,13 @@ __host__ __device__ constexpr int model_neq(int m) { return m == kTT2006 ? 19 : 
 // 27-point stencil slot of the neighbour offset (ox, oy, oz); 13 is the cell itself.
 __host__ __device__ constexpr int slot_of(int ox, int oy, int oz) { return (oz + 1) * 9 + (oy + 1) * 3 + (ox + 1); }
 
-// Device-resident CG scalars (one cache line per field group).
+// Device-resident CG scalars.  Written by block 0 of the kernel that
+// completes each reduction; r.z of iteration k lives in rrs[k & 1] so the
+// kernel that produces the new value never overwrites the one its other blocks
+// are still reading.
 struct CgScalars {
-    double rr;        // r.r of the current residual (or r.z with Jacobi)
-    double rr_true;   // r.r (always, for the stopping test)
+    double rrs[2];    // r.z (r.r without Jacobi) by iteration parity
+    double rr_true;   // r.r of the current residual (stopping test, reported error)
     double pAp;
     double alpha;
     double beta;

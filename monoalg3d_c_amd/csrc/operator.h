@@ -22,6 +22,28 @@ namespace mono {
 
 constexpr int kMaxSlots = 27;
 
+// Compile-time row patterns used by the tiled stencil kernels (diagonal first):
+// 7 = faces, 19 = faces + in-plane edges + corners (fibres in the xy plane),
+// 27 = the full neighbourhood.
+template <int P> struct RowPattern;
+template <> struct RowPattern<7> {
+    static constexpr int n = 7;
+    static constexpr int8_t o[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+};
+template <> struct RowPattern<19> {
+    static constexpr int n = 19;
+    static constexpr int8_t o[19][3] = {{0, 0, 0},   {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},   {0, 0, -1},
+                                        {1, 1, 0},   {-1, 1, 0},  {1, -1, 0},  {-1, -1, 0}, {-1, -1, -1}, {-1, -1, 1}, {-1, 1, -1},
+                                        {-1, 1, 1},  {1, -1, -1}, {1, -1, 1},  {1, 1, -1},  {1, 1, 1}};
+};
+template <> struct RowPattern<27> {
+    static constexpr int n = 27;
+    static constexpr int8_t o[27][3] = {{0, 0, 0},    {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},   {0, 0, -1},
+                                        {1, 1, 0},    {-1, 1, 0},  {1, -1, 0},  {-1, -1, 0}, {1, 0, 1},   {-1, 0, 1},  {1, 0, -1},
+                                        {-1, 0, -1},  {0, 1, 1},   {0, -1, 1},  {0, 1, -1},  {0, -1, -1}, {-1, -1, -1}, {-1, -1, 1},
+                                        {-1, 1, -1},  {-1, 1, 1},  {1, -1, -1}, {1, -1, 1},  {1, 1, -1},  {1, 1, 1}};
+};
+
 // how the conductivity tensor of a cell is given
 enum TensorSource : int {
     kTensorUniform = 0,   // one (sxx, syy, szz, sxy, sxz, syz) for all cells
@@ -40,34 +62,26 @@ struct AssemblyParams {
     const uint8_t *active;    // optional per-cell 0/1 (device)
 };
 
-// Device view of an operator used by the kernels.  Slot lists are padded to
-// the kernel's compile-time size with zero coefficients at offset 0, so row
-// products run without data-dependent branches.
+// Device view of an operator used by the kernels.  Interior and per-cell rows
+// are stored in the slot order of a compile-time RowPattern (diagonal first),
+// so the kernels address neighbours with immediate offsets; class rows keep all
+// 27 slots in 27-point order (slot = (oz+1)*9 + (oy+1)*3 + (ox+1)).
 template <typename T> struct OpView {
     int nx, ny, nz;
     int64_t n;
-    int nslots;                 // padded slot count S (7, 19 or 27)
-    int nint;                   // padded interior slot count NI (class mode)
-    int8_t off[kMaxSlots][3];   // neighbour offsets of the stored slots (slot 0 = diagonal)
-    int32_t lin[kMaxSlots];     // the same offsets as cell-index differences
-    int32_t lin_int[kMaxSlots]; // interior slot list (class mode)
-    const T *table;             // class mode: [(z*9 + cy*3 + cx) * S + k], else null
-    const T *table_int;         // class mode: [z * NI + k] (interior rows), else null
+    int pat;                    // pattern of table_int / percell (7, 19 or 27)
+    const T *table;             // class mode: [(z*9 + cy*3 + cx) * 27 + slot], else null
+    const T *table_int;         // class mode: [z * pat + k] interior rows, else null
     const T *percell;           // per-cell mode: [k * n + cell], else null
     const uint8_t *active;      // optional
 };
 
-// smallest kernel slot count that holds k slots
-inline int padded_slots(int k) { return k <= 7 ? 7 : (k <= 19 ? 19 : 27); }
-
 struct Operator {
     int nx = 0, ny = 0, nz = 0;
     int64_t n = 0;
-    int nslots = 0;        // distinct stored slots (union over all rows)
-    int S = 0;             // padded
-    int nint = 0, NI = 0;  // interior rows' slot union and its padded size
-    int8_t off[kMaxSlots][3] = {};
-    int8_t off_int[kMaxSlots][3] = {};
+    int nslots = 0;        // distinct non-zero slots (union over all rows)
+    int nint = 0;          // class mode: distinct slots of the interior rows
+    int pat = 27;          // pattern of the interior rows (class mode) or of all rows (per-cell)
     bool class_mode = false;
     double *table64 = nullptr, *tint64 = nullptr;
     float *table32 = nullptr, *tint32 = nullptr;

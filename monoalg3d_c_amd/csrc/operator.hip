@@ -16,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "mono_common.h"
@@ -399,13 +400,7 @@ static OpView<T> make_view(const Operator &o, const T *table, const T *tint, con
     v.ny = o.ny;
     v.nz = o.nz;
     v.n = o.n;
-    v.nslots = o.S;
-    v.nint = o.NI;
-    memcpy(v.off, o.off, sizeof(v.off));
-    for(int k = 0; k < kMaxSlots; k++) {
-        v.lin[k] = (int32_t)(o.off[k][0] + (int64_t)o.nx * (o.off[k][1] + (int64_t)o.ny * o.off[k][2]));
-        v.lin_int[k] = (int32_t)(o.off_int[k][0] + (int64_t)o.nx * (o.off_int[k][1] + (int64_t)o.ny * o.off_int[k][2]));
-    }
+    v.pat = o.pat;
     v.table = table;
     v.table_int = tint;
     v.percell = percell;
@@ -429,13 +424,31 @@ int Operator::release() {
     return 0;
 }
 
-static void slot_offsets(const std::vector<int> &slots, int8_t (*off)[3]) {
-    for(int k = 0; k < kMaxSlots; k++) {
-        const int sl = k < (int)slots.size() ? slots[k] : 13;  // padding: the cell itself, coefficient 0
-        off[k][0] = (int8_t)(sl % 3 - 1);
-        off[k][1] = (int8_t)((sl / 3) % 3 - 1);
-        off[k][2] = (int8_t)(sl / 9 - 1);
+// slots (27-point numbering) of a row pattern
+template <int P> static std::vector<int> pattern_slots() {
+    std::vector<int> v;
+    for(int k = 0; k < P; k++) v.push_back(slot_of(RowPattern<P>::o[k][0], RowPattern<P>::o[k][1], RowPattern<P>::o[k][2]));
+    return v;
+}
+
+// smallest pattern containing every slot of `used`
+static int choose_pattern(const std::vector<int> &used, std::vector<int> *slots) {
+    const std::vector<int> p7 = pattern_slots<7>(), p19 = pattern_slots<19>(), p27 = pattern_slots<27>();
+    auto covers = [&](const std::vector<int> &pat) {
+        for(int u : used)
+            if(std::find(pat.begin(), pat.end(), u) == pat.end()) return false;
+        return true;
+    };
+    if(covers(p7)) {
+        *slots = p7;
+        return 7;
     }
+    if(covers(p19)) {
+        *slots = p19;
+        return 19;
+    }
+    *slots = p27;
+    return 27;
 }
 
 int build_operator(const AssemblyParams &p, const double *coef27, Operator *op, hipStream_t s) {
@@ -463,8 +476,8 @@ int build_operator(const AssemblyParams &p, const double *coef27, Operator *op, 
     for(int k = 0; k < 27; k++)
         if(k != 13 && (mask >> k) & 1u) slots.push_back(k);
     op->nslots = (int)slots.size();
-    op->S = padded_slots(op->nslots);
-    slot_offsets(slots, op->off);
+    std::vector<int> bslots;  // the smallest pattern covering every row (per-cell storage order)
+    const int pb = choose_pattern(slots, &bslots);
 
     // class table candidate: representative cell of each (z, cy, cx) class
     const int ncls = p.nz * 9;
@@ -517,19 +530,15 @@ int build_operator(const AssemblyParams &p, const double *coef27, Operator *op, 
             if(used) islots.push_back(k);
         }
         op->nint = (int)islots.size();
-        op->NI = padded_slots(op->nint);
-        slot_offsets(islots, op->off_int);
-        const int S = op->S, NI = op->NI;
-        std::vector<double> t64((size_t)ncls * S, 0.0), i64((size_t)p.nz * NI, 0.0);
-        std::vector<float> t32((size_t)ncls * S, 0.0f), i32((size_t)p.nz * NI, 0.0f);
-        for(int c = 0; c < ncls; c++)
-            for(int k = 0; k < op->nslots; k++) {
-                t64[(size_t)c * S + k] = table27[(size_t)c * 27 + slots[k]];
-                t32[(size_t)c * S + k] = (float)table27[(size_t)c * 27 + slots[k]];
-            }
+        std::vector<int> pslots;
+        op->pat = choose_pattern(islots, &pslots);
+        const int NI = op->pat;
+        std::vector<double> t64(table27), i64((size_t)p.nz * NI, 0.0);
+        std::vector<float> t32((size_t)ncls * 27, 0.0f), i32((size_t)p.nz * NI, 0.0f);
+        for(size_t i = 0; i < t64.size(); i++) t32[i] = (float)t64[i];
         for(int z = 0; z < p.nz; z++)
-            for(int k = 0; k < op->nint; k++) {
-                const double v = table27[((size_t)(z * 3 + cyF) * 3 + cxF) * 27 + islots[k]];
+            for(int k = 0; k < NI; k++) {
+                const double v = table27[((size_t)(z * 3 + cyF) * 3 + cxF) * 27 + pslots[k]];
                 i64[(size_t)z * NI + k] = v;
                 i32[(size_t)z * NI + k] = (float)v;
             }
@@ -543,15 +552,15 @@ int build_operator(const AssemblyParams &p, const double *coef27, Operator *op, 
         MONO_CHECK(hipMemcpyAsync(op->tint32, i32.data(), i32.size() * sizeof(float), hipMemcpyHostToDevice, s));
         MONO_CHECK(hipStreamSynchronize(s));
     } else {
-        op->nint = op->NI = 0;
-        std::vector<int> padded(op->S, -1);
-        for(int k = 0; k < op->nslots; k++) padded[k] = slots[k];
+        op->nint = 0;
+        std::vector<int> pslots = bslots;
+        op->pat = pb;
         int *dslots = nullptr;
-        MONO_CHECK(hipMalloc(&dslots, sizeof(int) * op->S));
-        MONO_CHECK(hipMemcpyAsync(dslots, padded.data(), sizeof(int) * op->S, hipMemcpyHostToDevice, s));
-        MONO_CHECK(hipMalloc(&op->percell64, sizeof(double) * op->S * n));
-        MONO_CHECK(hipMalloc(&op->percell32, sizeof(float) * op->S * n));
-        k_compact<<<grid, 256, 0, s>>>(coef27, n, op->S, dslots, op->percell64, op->percell32);
+        MONO_CHECK(hipMalloc(&dslots, sizeof(int) * op->pat));
+        MONO_CHECK(hipMemcpyAsync(dslots, pslots.data(), sizeof(int) * op->pat, hipMemcpyHostToDevice, s));
+        MONO_CHECK(hipMalloc(&op->percell64, sizeof(double) * op->pat * n));
+        MONO_CHECK(hipMalloc(&op->percell32, sizeof(float) * op->pat * n));
+        k_compact<<<grid, 256, 0, s>>>(coef27, n, op->pat, dslots, op->percell64, op->percell32);
         MONO_CHECK(hipGetLastError());
         MONO_CHECK(hipStreamSynchronize(s));
         MONO_CHECK(hipFree(dslots));

@@ -1,0 +1,318 @@
+#!/usr/bin/env python3
+"""Generate the golden fixtures in tests/golden/ from the reference itself.
+
+Runs the reference's CPU build made by oracle/ref.mk (oracle/_ref: the
+MonoAlg3D binary and its plugins, built from /root/reference sources) and the
+harness oracle/ref_harness.c, which drives the reference plugins through their
+own dlopen ABI.  Writes small .npz fixtures plus provenance.json.  Needs the
+reference sources (this container); the fixtures themselves are data and
+travel with the repository.
+
+    python tests/golden/make_golden.py            # all fixtures
+    python tests/golden/make_golden.py ionic      # one group
+"""
+from __future__ import annotations
+
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref")
+DT = 0.02
+H = 100.0
+PROV = {}
+
+MODELS = {
+    "tt2006": ("libten_tusscher_2006.so", -50.0, 19),
+    "ord2011": ("libohara_rudy_endo_2011.so", -50.0, 41),
+    "ms2003": ("libmitchell_shaeffer_2003.so", 0.2, 2),
+}
+
+
+def run(cmd, cwd=REF):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{cmd} failed:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    return r.stdout
+
+
+def stim_window(nsteps):
+    """Steps whose accumulated time lies in [0, 2] ms (ode_solver.c:319-327)."""
+    on, t = np.zeros(nsteps, bool), 0.0
+    for k in range(nsteps):
+        on[k] = 0.0 <= t <= 2.0
+        t += DT
+    return on
+
+
+def ionic(tmp):
+    """G1: action-potential traces; G2: one-step input/output pairs."""
+    for name, (lib, amp, neq) in MODELS.items():
+        nsteps, rec = 25000, 50
+        amps = np.array([amp, 0.6 * amp])
+        n = len(amps)
+        stim = np.where(stim_window(nsteps)[:, None], amps[None, :], 0.0)
+        sp, op = os.path.join(tmp, "stim.bin"), os.path.join(tmp, "out.bin")
+        stim.tofile(sp)
+        run([os.path.join(REF, "bin", "ref_harness"), "ionic", "./shared_libs/" + lib, str(n), str(nsteps), str(DT), "-", sp, op,
+             str(rec)])
+        trace = np.fromfile(op).reshape(nsteps // rec, n, neq)
+        # one-step pairs from states along a trace recorded every 97 steps
+        run([os.path.join(REF, "bin", "ref_harness"), "ionic", "./shared_libs/" + lib, str(n), str(nsteps), str(DT), "-", sp, op,
+             "97"])
+        states = np.fromfile(op).reshape(-1, n, neq).reshape(-1, neq)[:200]
+        rng = np.random.default_rng(11)
+        s1 = np.where(rng.random(len(states)) < 0.5, amp, 0.0)
+        st_in = os.path.join(tmp, "states.bin")
+        states.tofile(st_in)
+        s1.tofile(sp)
+        run([os.path.join(REF, "bin", "ref_harness"), "ionic", "./shared_libs/" + lib, str(len(states)), "1", str(DT), st_in, sp, op])
+        after = np.fromfile(op).reshape(len(states), neq)
+        np.savez_compressed(os.path.join(HERE, f"ionic_{name}.npz"), trace=trace, trace_every=rec, trace_amps=amps,
+                            pair_in=states, pair_stim=s1, pair_out=after, dt=DT)
+        PROV[f"ionic_{name}.npz"] = (f"ref_harness ionic on {lib}: {n} cells, {nsteps} steps of {DT} ms, stimulus amps "
+                                     f"{amps.tolist()} for t in [0,2] ms, states every {rec} steps; 200 one-step pairs")
+
+
+def fibres_layered(shape, th0=-60.0, th1=60.0):
+    nx, ny, nz = shape
+    f = np.zeros((nx * ny * nz, 3))
+    for c in range(nx * ny * nz):
+        z = c // (nx * ny)
+        th = np.deg2rad(th0 + (th1 - th0) * z / (nz - 1))
+        v = np.array([np.cos(th), np.sin(th), 0.0])
+        m = np.sqrt((v[0] * v[0]) + (v[1] * v[1]) + (v[2] * v[2]))
+        f[c] = v / m
+    return f
+
+
+def read_rows(path, n):
+    raw = open(path, "rb").read()
+    A = np.zeros((n, 27))
+    off = 0
+    import struct
+    nx_, ny_ = read_rows.shape[0], read_rows.shape[1]
+    while off < len(raw):
+        row, ne = struct.unpack_from("<ii", raw, off)
+        off += 8
+        rx, ry, rz = row % nx_, (row // nx_) % ny_, row // (nx_ * ny_)
+        for _ in range(ne):
+            col, v = struct.unpack_from("<id", raw, off)
+            off += 12
+            cx, cy, cz = col % nx_, (col // nx_) % ny_, col // (nx_ * ny_)
+            slot = (cz - rz + 1) * 9 + (cy - ry + 1) * 3 + (cx - rx + 1)
+            A[row, slot] += v
+    return A
+
+
+def assembly(tmp):
+    """G4: assembled rows of the reference's assembly plugins."""
+    shape = (7, 6, 5)
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    read_rows.shape = shape
+    rng = np.random.default_rng(12)
+    out = {}
+    for holes in (False, True):
+        act = np.ones(n, np.uint8)
+        if holes:
+            act[rng.random(n) < 0.15] = 0
+        fib = fibres_layered(shape)
+        cells = os.path.join(tmp, "cells.bin")
+        with open(cells, "wb") as fh:
+            for c in range(n):
+                fh.write(bytes([act[c]]))
+                fh.write(fib[c].tobytes())
+        tag = "holes" if holes else "box"
+        ip = os.path.join(tmp, "iso.bin")
+        run([os.path.join(REF, "bin", "ref_harness"), "assemble", "iso", str(nx), str(ny), str(nz), str(H), cells, ip, "0.00013"])
+        out[f"iso_{tag}"] = read_rows(ip, n)
+        ap = os.path.join(tmp, "an.bin")
+        run([os.path.join(REF, "bin", "ref_harness"), "assemble", "aniso", str(nx), str(ny), str(nz), str(H), cells, ap, "0.00024",
+             "0.000045", "0.000045"])
+        out[f"aniso_{tag}"] = read_rows(ap, n)
+        out[f"active_{tag}"] = act
+    out["fibres"] = fibres_layered(shape)
+    np.savez_compressed(os.path.join(HERE, "assembly.npz"), shape=np.array(shape), **out)
+    PROV["assembly.npz"] = ("ref_harness assemble (homogeneous_sigma_assembly_matrix sigma 0.00013; "
+                            "anisotropic_sigma_assembly_matrix 0.00024/0.000045 with per-cell fibres rotating -60..60 deg "
+                            "over z) on a 7x6x5 cuboid, with and without 15% inactive cells; rows as 27-point stencils")
+
+
+def ini_text(out_dir, T, model_lib, domain, assembly_sec, print_rate):
+    return f"""[main]
+num_threads = 8
+dt_pde = {DT}
+simulation_time = {T}
+abort_on_no_activity = false
+use_adaptivity = false
+[update_monodomain]
+main_function=update_monodomain_default
+[save_result]
+print_rate={print_rate}
+output_dir={out_dir}
+main_function=save_as_text_or_binary
+file_prefix=V
+binary=true
+[assembly_matrix]
+init_function=set_initial_conditions_fvm
+{assembly_sec}
+[linear_system_solver]
+tolerance=1e-16
+use_preconditioner=no
+max_iterations=200
+main_function=conjugate_gradient
+[domain]
+{domain}
+[ode_solver]
+dt={DT}
+use_gpu=no
+library_file=shared_libs/{model_lib}
+[stim_plain]
+start = 0.0
+duration = 2.0
+current = -50.0
+x_limit = 500.0
+main_function=stim_if_x_less_than
+"""
+
+
+def read_snapshots(out_dir, shape, every, nsteps):
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    snaps = []
+    for k in range(0, nsteps, every):
+        rec = np.fromfile(os.path.join(out_dir, f"V_it_{k}.bin")).reshape(-1, 7)
+        lex = (np.floor(rec[:, 0] / H) + nx * (np.floor(rec[:, 1] / H) + ny * np.floor(rec[:, 2] / H))).astype(int)
+        v = np.full(n, np.nan)
+        v[lex] = rec[:, 6]
+        snaps.append(v)
+    return np.array(snaps)
+
+
+def lat_from(snaps, every, thr=-30.0):
+    """First upward crossing of thr, linearly interpolated between samples (ms)."""
+    lat = np.full(snaps.shape[1], np.inf)
+    for k in range(1, len(snaps)):
+        a, b = snaps[k - 1], snaps[k]
+        hit = np.isinf(lat) & (a < thr) & (b >= thr)
+        tt = (k - 1 + (thr - a[hit]) / (b[hit] - a[hit])) * every * DT
+        lat[hit] = tt
+    return lat
+
+
+def tissue(tmp, name, shape, model, T, every, aniso=False, fibrosis=0.0, keep_times=()):
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    out_dir = os.path.join(tmp, name)
+    os.makedirs(out_dir, exist_ok=True)
+    active = np.ones(n, np.uint8)
+    if not aniso:
+        domain = (f"name=Cuboid Mesh\nstart_dx={H}\nstart_dy={H}\nstart_dz={H}\nside_length_x={nx * H}\n"
+                  f"side_length_y={ny * H}\nside_length_z={nz * H}\nmain_function=initialize_grid_with_cuboid_mesh")
+        asm = ("sigma_x=0.00013\nsigma_y=0.00013\nsigma_z=0.00013\n"
+               "main_function=homogeneous_sigma_assembly_matrix")
+        fib = None
+    else:
+        rng = np.random.default_rng(13)
+        if fibrosis > 0:
+            active[rng.random(n) < fibrosis] = 0
+        fib = fibres_layered(shape)
+        mesh = os.path.join(out_dir, "mesh.alg")
+        fibf = os.path.join(out_dir, "fibres.txt")
+        idx = np.nonzero(active)[0]
+        with open(mesh, "w") as fm, open(fibf, "w") as ff:
+            for c in idx:
+                x, y, z = c % nx, (c // nx) % ny, c // (nx * ny)
+                fm.write(f"{(x + 0.5) * H:.1f},{(y + 0.5) * H:.1f},{(z + 0.5) * H:.1f},{H / 2},{H / 2},{H / 2}\n")
+                f0 = fib[c]
+                ff.write("%.17g %.17g %.17g %.17g %.17g 0 0 0 1\n" % (f0[0], f0[1], f0[2], -f0[1], f0[0]))
+        domain = (f"name=Custom Mesh\nmain_function=initialize_grid_with_custom_mesh\nmesh_file={mesh}\n"
+                  f"number_of_points={len(idx)}\nstart_discretization={H}")
+        asm = ("sigma_l=0.00024\nsigma_t=0.000045\nsigma_n=0.000045\n"
+               f"fibers_file={fibf}\nmain_function=anisotropic_sigma_assembly_matrix")
+    lib = MODELS[model][0]
+    ini = os.path.join(out_dir, "run.ini")
+    with open(ini, "w") as fh:
+        fh.write(ini_text(out_dir, T, lib, domain, asm, every))
+    log = run([os.path.join(REF, "bin", "MonoAlg3D"), "-c", ini])
+    nsteps = int(round(T / DT)) + 1
+    snaps = read_snapshots(out_dir, shape, every, nsteps)
+    its = [int(line.split("Iterations = ")[1].split(",")[0]) for line in log.splitlines() if line.startswith("t = ")]
+    fx = {"shape": np.array(shape), "every": every, "active": active, "lat": lat_from(snaps, every),
+          "cg_iterations_every_1ms": np.array(its)}
+    for tms in keep_times:
+        fx[f"vm_t{int(tms)}"] = snaps[int(round(tms / (every * DT)))]
+    probes = np.linspace(0, n - 1, 12).astype(int)
+    fx["probe_cells"] = probes
+    fx["probe_vm"] = snaps[:, probes]
+    if fib is not None:
+        fx["fibres"] = fib
+    np.savez_compressed(os.path.join(HERE, f"tissue_{name}.npz"), **fx)
+    PROV[f"tissue_{name}.npz"] = (f"reference MonoAlg3D (oracle/_ref) {nx}x{ny}x{nz} cells at {H} um, {model}, "
+                                  f"{'anisotropic 0.24/0.045 S/m, fibres -60..60 deg over z (custom mesh + fibres file)' if aniso else 'isotropic 0.13 S/m'}"
+                                  f"{f', {fibrosis:.0%} cells omitted (inactive)' if fibrosis else ''}; stimulus -50 on x<500 um for "
+                                  f"t in [0,2] ms; CG tol 1e-16; {T} ms; Vm saved every {every} steps; LAT = first crossing of "
+                                  f"-30 mV interpolated between saved samples")
+
+
+def read_octave(path):
+    """Octave text matrix (sparse: 'row col value' 1-based; dense: one value per line)."""
+    lines = open(path).read().splitlines()
+    data = [ln for ln in lines if ln and not ln.startswith("#")]
+    if any("sparse" in ln for ln in lines[:5]):
+        a = np.array([[float(t) for t in ln.split()] for ln in data])
+        return a[:, 0].astype(np.int32) - 1, a[:, 1].astype(np.int32) - 1, a[:, 2]
+    return np.array([float(ln) for ln in data])
+
+
+def solvers(tmp):
+    """The reference's own CG test systems (tests_bin/A1-A3, B1-B3, X1-X3; test_solvers.c:28-111)."""
+    src = "/root/reference/tests_bin"
+    out = {}
+    for k in (1, 2, 3):
+        r, c, v = read_octave(os.path.join(src, f"A{k}.txt"))
+        out[f"A{k}_row"], out[f"A{k}_col"], out[f"A{k}_val"] = r, c, v
+        out[f"B{k}"] = read_octave(os.path.join(src, f"B{k}.txt"))
+        out[f"X{k}"] = read_octave(os.path.join(src, f"X{k}.txt"))
+    np.savez_compressed(os.path.join(HERE, "solver_systems.npz"), **out)
+    PROV["solver_systems.npz"] = ("tests_bin/A1-A3, B1-B3, X1-X3 of the reference (Octave SPD systems used by "
+                                  "test_solvers.c with a 1e-3 absolute tolerance), converted to COO arrays")
+
+
+def main():
+    groups = sys.argv[1:] or ["ionic", "assembly", "tissue", "solvers"]
+    if not os.path.exists(os.path.join(REF, "bin", "ref_harness")):
+        raise SystemExit("build the reference first: make -f oracle/ref.mk")
+    prov_path = os.path.join(HERE, "provenance.json")
+    if os.path.exists(prov_path):
+        PROV.update(json.load(open(prov_path)))
+    tmp = tempfile.mkdtemp(prefix="golden_", dir=os.path.join(ROOT, "_work") if os.path.isdir(os.path.join(ROOT, "_work")) else None)
+    try:
+        if "ionic" in groups:
+            ionic(tmp)
+        if "assembly" in groups:
+            assembly(tmp)
+        if "tissue" in groups:
+            tissue(tmp, "iso_tt2006_32x32x8", (32, 32, 8), "tt2006", 20.0, 5, keep_times=(10, 20))
+            tissue(tmp, "aniso_ord_24x24x6", (24, 24, 6), "ord2011", 10.0, 5, aniso=True, keep_times=(5, 10))
+            tissue(tmp, "fibrotic_ord_24x24x6", (24, 24, 6), "ord2011", 10.0, 5, aniso=True, fibrosis=0.1, keep_times=(5, 10))
+        if "solvers" in groups:
+            solvers(tmp)
+        if "config1" in groups:
+            tissue(tmp, "config1_tt2006_64x64x16", (64, 64, 16), "tt2006", 50.0, 10, keep_times=(20,))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    with open(prov_path, "w") as fh:
+        json.dump(PROV, fh, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()
