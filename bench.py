@@ -59,6 +59,8 @@ def bytes_per_cell_step(neq, w_state, k_iter, mixed, outer):
 
 
 def run_precision(shape, precision, steps, warmup):
+    """Warm up, time `steps` steps (wall clock around the whole step, device
+    resident), then time up to 10 more steps per phase with HIP events."""
     import monoalg3d_c_amd as m
     n = shape[0] * shape[1] * shape[2]
     t0 = time.time()
@@ -67,21 +69,66 @@ def run_precision(shape, precision, steps, warmup):
     tis.step(warmup)
     tis.sync()
     s0 = tis.stats
-    tis.set_timing(True)
     t0 = time.perf_counter()
     tis.step(steps)
     tis.sync()
     wall = time.perf_counter() - t0
     s1 = tis.stats
+    k2 = min(steps, 10)
+    tis.set_timing(True)
+    tis.step(k2)
+    tis.sync()
+    s2 = tis.stats
     tis.close()
     its = (s1["cg_iterations_total"] - s0["cg_iterations_total"]) / steps
     inner = (s1["inner_iterations_total"] - s0["inner_iterations_total"]) / steps
     outer = (s1["outer_sweeps_total"] - s0["outer_sweeps_total"]) / steps
-    ionic_ms = (s1["ionic_ms_total"] - s0["ionic_ms_total"]) / steps
-    solve_ms = (s1["solve_ms_total"] - s0["solve_ms_total"]) / steps
+    ionic_ms = (s2["ionic_ms_total"] - s1["ionic_ms_total"]) / k2
+    solve_ms = (s2["solve_ms_total"] - s1["solve_ms_total"]) / k2
     return dict(rate=n * steps / wall, ms_per_step=1e3 * wall / steps, cg_its=its, inner_its=inner, outer=outer,
                 ionic_ms=ionic_ms, solve_ms=solve_ms, setup_s=setup_s, class_operator=s1["class_operator"],
                 stencil_points=s1["stencil_points"])
+
+
+def lat_run(shape, precision, lat_ms):
+    """Untimed: a fresh run to lat_ms with the activation observer on every step
+    (save_with_activation_times with print_rate 1); returns first activation times."""
+    import monoalg3d_c_amd as m
+    tis = m.Tissue(shape, **workload_desc(shape, precision))
+    tis.observe_setup(-30.0, -83.0, 10.0, 1)
+    for _ in range(int(round(lat_ms / 0.02)) + 1):
+        tis.observe()
+        tis.step(1)
+    a = tis.activations()
+    tis.close()
+    lat = a["act"][:, 0].astype(np.float64)
+    lat[a["n_act"] == 0] = np.inf
+    return lat
+
+
+def lat_error(lat64, lat32):
+    act64, act32 = np.isfinite(lat64), np.isfinite(lat32)
+    both = act64 & act32
+    return dict(activated_fp64=int(act64.sum()), activated_fp32=int(act32.sum()),
+                activated_in_one_only=int((act64 ^ act32).sum()),
+                max_abs_err_ms=float(np.abs(lat64[both] - lat32[both]).max()) if both.any() else None)
+
+
+def measured_traffic(kernel, shape):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, each
+    scaled by the same counters on a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        e = d["kernels"][kernel]
+        if tuple(d.get("shape", ())) != tuple(shape):
+            return None
+        return int(e["bytes_per_launch"])
+    except (KeyError, ValueError, TypeError):
+        return None
 
 
 def cpu_baseline(max_threads=16):
@@ -167,6 +214,10 @@ def main() -> int:
     ap.add_argument("--shape", type=str, default="400,400,100")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32", "both"], default="both")
+    ap.add_argument("--lat-ms", type=float, default=20.0,
+                    help="simulated ms of the untimed FP64 vs FP32+refine activation-time check (0: skip)")
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="also run a device copy of this many MiB (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is a design "
@@ -188,6 +239,22 @@ def main() -> int:
         print(f"[bench] fp32+refine {res['fp32']}", file=sys.stderr, flush=True)
     head = res.get("fp32") or res["fp64"]
     mixed = "fp32" in res
+    lat_check = None
+    if args.lat_ms > 0 and len(res) == 2:
+        lat64 = lat_run(shape, m.FP64, args.lat_ms)
+        lat32 = lat_run(shape, m.FP32_REFINE, args.lat_ms)
+        lat_check = dict(simulated_ms=args.lat_ms, **lat_error(lat64, lat32))
+        lat_check["within_0p5_ms"] = (lat_check["max_abs_err_ms"] is not None and lat_check["max_abs_err_ms"] <= 0.5
+                                      and lat_check["activated_in_one_only"] == 0)
+        print(f"[bench] lat check {lat_check}", file=sys.stderr, flush=True)
+    if args.copy_calib_mb > 0:
+        import torch
+        nbytes = args.copy_calib_mb << 20
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
 
     # roofline of the dominant phase (by device time)
     rf = {}
@@ -204,9 +271,12 @@ def main() -> int:
     hk = "fp32" if mixed else "fp64"
     dom = "ionic" if res[hk]["ionic_ms"] >= res[hk]["solve_ms"] else "solve"
     d = rf[hk][dom]
+    kname = ("k_ionic_step<ORd2011," + ("float" if hk == "fp32" else "double") + ">") if dom == "ionic" else \
+        f"diffusion solve ({hk})"
     roofline = {"bound": "hbm", "achieved": round(d["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(d["achieved"] / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"{dom} phase ({hk})"}
+                "frac": round(d["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname, shape),
+                "kernel": kname, "launch_ms": round(res[hk][dom + "_ms"], 4),
+                "bytes_per_launch": int(d["bytes"])}
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": METRIC,
@@ -219,7 +289,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64refine" if mixed else "f64",
+        "dtype": "f32" if mixed else "f64",
         "data": "synthetic",
         "config": {"workload": f"cuboid {shape[0]}x{shape[1]}x{shape[2]} cells at 100 um, ORd 2011 endo, "
                                "anisotropic 0.24/0.045 S/m fibres -60..+60 deg over depth, planar stimulus, "
@@ -231,6 +301,7 @@ def main() -> int:
         "fp32_refine": res.get("fp32"),
         "speedup_fp32_refine_vs_fp64": (res["fp32"]["rate"] / res["fp64"]["rate"]) if len(res) == 2 else None,
         "phase_roofline": rf,
+        "lat_check": lat_check,
     }
     print(json.dumps(out))
     return 0

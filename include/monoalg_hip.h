@@ -22,6 +22,8 @@
  *   mono_tissue_get_state   (save_mesh.c:248-420) and the SV dump of
  *                           save_state.c:16-155.
  *   mono_tissue_destroy     END_LINEAR_SYSTEM + the SV free of ode_solver.c:149-158.
+ *   mono_tissue_observe     the activation-time / APD bookkeeping of
+ *                           save_with_activation_times (save_mesh.c:730-913).
  *   mono_ionic_run          SET_ODE_INITIAL_CONDITIONS_GPU + SOLVE_MODEL_ODES on
  *                           caller-given states and per-cell stimuli
  *                           (ode_solver.h:37-47; the GPU plugins'
@@ -135,6 +137,18 @@ int mono_tissue_set_timing(mono_tissue *t, int on);
 /* device pointers for zero-copy consumers: v (FP64, n), sv (neq rows of
  * `pitch` elements, FP64 or FP32 by precision), b (FP64, n) */
 int mono_tissue_device_ptrs(mono_tissue *t, void **v, void **sv, size_t *pitch, void **b);
+
+/* Activation times and APDs (save_with_activation_times, save_mesh.c:730-913),
+ * kept on the device.  Call mono_tissue_observe where the reference calls its
+ * save function: every print_rate iterations, before mono_tissue_step, the
+ * first time at t = 0.  Thresholds in mV and ms (reference defaults -30, -83,
+ * 10); up to max_events activation times and APDs are kept per cell. */
+int mono_tissue_observe_setup(mono_tissue *t, float activation_threshold, float apd_threshold, float time_threshold,
+                              int max_events);
+int mono_tissue_observe(mono_tissue *t);
+/* n_act, n_apd: n counts; act, apd: max_events x n floats (row k = k-th event,
+ * NaN where absent).  Any pointer may be NULL. */
+int mono_tissue_get_activations(mono_tissue *t, int32_t *n_act, float *act, int32_t *n_apd, float *apd);
 
 /* Standalone ionic solve: n cells, states AoS (n x neq, FP64, in/out), per-cell
  * stimulus (n doubles, may be NULL), nsteps ODE steps of dt.  precision selects

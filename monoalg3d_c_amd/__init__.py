@@ -175,6 +175,47 @@ class Tissue:
               "mono_tissue_device_ptrs")
         return v.value, sv.value, pitch.value, b.value
 
+    # activation times and APDs (save_with_activation_times, save_mesh.c:730-913)
+    def observe_setup(self, activation_threshold=-30.0, apd_threshold=-83.0, time_threshold=10.0,
+                      max_events=8) -> None:
+        check(lib().mono_tissue_observe_setup(self._h, activation_threshold, apd_threshold, time_threshold,
+                                              max_events), "mono_tissue_observe_setup")
+        self._max_events = max_events
+
+    def observe(self) -> None:
+        """One save call at the current time (every print_rate steps, before stepping)."""
+        check(lib().mono_tissue_observe(self._h), "mono_tissue_observe")
+
+    def activations(self) -> dict:
+        n, k = self.n, self._max_events
+        n_act = np.zeros(n, np.int32)
+        n_apd = np.zeros(n, np.int32)
+        act = np.zeros((k, n), np.float32)
+        apd = np.zeros((k, n), np.float32)
+        check(lib().mono_tissue_get_activations(self._h, n_act.ctypes.data, act.ctypes.data, n_apd.ctypes.data,
+                                                apd.ctypes.data), "mono_tissue_get_activations")
+        return {"n_act": n_act, "act": act.T.copy(), "n_apd": n_apd, "apd": apd.T.copy()}
+
+
+def write_acm(path, shape, h, n_act, act, apd, last_call=True, active=None) -> None:
+    """The reference's activation_info .acm text (save_mesh.c:789-905): a flag
+    line, then per active cell 'cx,cy,cz,hx,hy,hz n [ t.. ] [ apd.. ]' (cells in
+    lexicographic order here; readers key cells by their centre)."""
+    nx, ny, nz = shape
+    dx, dy, dz = h
+    with open(path, "w") as fh:
+        fh.write(f"{int(bool(last_call))}\n")
+        for c in range(nx * ny * nz):
+            if active is not None and not active[c]:
+                continue
+            x, y, z = c % nx, (c // nx) % ny, c // (nx * ny)
+            cx, cy, cz = (x + 0.5) * dx, (y + 0.5) * dy, (z + 0.5) * dz
+            na = int(n_act[c])
+            ts = " ".join("%lf" % v for v in act[c][:na] if np.isfinite(v))
+            ap = " ".join("%lf" % v for v in apd[c] if np.isfinite(v))
+            fh.write("%g,%g,%g,%g,%g,%g %d [ %s%s] [ %s%s]\n" % (cx, cy, cz, dx / 2.0, dy / 2.0, dz / 2.0, na, ts,
+                                                                  " " if ts else "", ap, " " if ap else ""))
+
 
 def ionic_run(model: int, states: np.ndarray, stim: np.ndarray | None, dt: float, nsteps: int = 1,
               precision: int = FP64) -> np.ndarray:

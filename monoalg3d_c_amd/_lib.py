@@ -24,6 +24,7 @@ EXPORTED = (
     "mono_tissue_get_state", "mono_tissue_set_state", "mono_tissue_get_stats", "mono_tissue_set_timing",
     "mono_tissue_device_ptrs", "mono_ionic_run", "mono_model_neq", "mono_model_initial_state",
     "mono_model_initial_v", "mono_assemble", "mono_cg_stencil", "mono_cg_csr",
+    "mono_tissue_observe_setup", "mono_tissue_observe", "mono_tissue_get_activations",
 )
 
 
@@ -97,6 +98,9 @@ def lib() -> C.CDLL:
     L.mono_assemble.argtypes = [C.POINTER(TissueDesc), vp, dp]
     L.mono_cg_stencil.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
                                   C.c_int, C.POINTER(C.c_int), dp]
+    L.mono_tissue_observe_setup.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_int]
+    L.mono_tissue_observe.argtypes = [vp]
+    L.mono_tissue_get_activations.argtypes = [vp, vp, vp, vp, vp]
     L.mono_cg_csr.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
     _lib = L
     return L

@@ -19,7 +19,9 @@
 // Mixed precision (FP32 CG with FP64 defect correction): the outer residual
 // r = b - A x is formed in FP64 with the FP64 operator; an FP32 CG solves
 // A d = r to a relative tolerance with FP64 dot accumulation; x += d in FP64.
+#include <stdlib.h>
 #include <type_traits>
+#include <vector>
 
 #include "cg.h"
 
@@ -198,18 +200,17 @@ template <typename TC> __device__ __forceinline__ double diag_of(const OpView<TC
 // ---------------------------------------------------------------------------
 // Tiled stencil sweep.  A block owns a 64 x 8 tile of an xy plane and marches
 // through z planes, keeping planes z-1, z, z+1 of the tile (plus a one-cell
-// halo, zero outside the box) in LDS; the next plane's global loads are issued
-// before the current plane is computed.  Rows follow compile-time patterns, so
-// every LDS read uses an immediate offset.
-//   per-cell mode: tiles cover the box; every cell reads its own row
-//     (pattern PAT) and the zero halo stands in for missing neighbours;
-//   class mode: tiles cover the cells whose x and y lie in the axes' interior
-//     ranges, so a plane's row is the same for every lane: the layer's
-//     interior row (pattern PAT, scalar registers) or, on the two boundary
-//     planes, the 27-slot class row.  The remaining cells (the x and y faces
-//     of the box, about 4/n of an n x n plane) are evaluated by "edge" blocks
-//     of the same launch from global memory.
-// Work items are (xy tile, z plane) pairs in tile-major order; each tile block
+// halo, zero outside the box) in LDS; the loads of planes z+2 and z+3 are in
+// flight while plane z is computed.  Rows follow compile-time patterns, so
+// every LDS read uses an immediate offset.  A wave computes one 64-cell row
+// segment of the tile.
+//   per-cell mode: every cell reads its own row (pattern PAT);
+//   class mode: interior cells use their layer's interior row (pattern PAT,
+//     wave-uniform, in scalar registers); rows on a y face and the two
+//     boundary planes use the 27-slot row of their class (also wave-uniform);
+//     the cells on an x face (at most 16 per tile plane) are left out of the
+//     wave-wide pass and evaluated by 16 lanes of wave 0 with per-lane rows.
+// Work items are (xy tile, z plane) pairs in tile-major order; each block
 // takes one contiguous run of `per` items (a few tiles' z ranges at most), and
 // the grid is one wave of resident blocks, so all blocks finish together.
 // With FUSE the value staged for the stencil is the new search direction
@@ -221,16 +222,9 @@ constexpr int kNLD = (kPL + kBlock - 1) / kBlock;
 constexpr int kRPT = kTX * kTY / kBlock;  // rows per thread
 
 struct TileGeom {
-    int ox, oy, lx, ly;         // tiled cells: x in [ox, ox+lx), y in [oy, oy+ly)
-    int ntx, nty;
-    int nedge_blocks, nblocks;  // edge blocks first, then tile blocks
-    int64_t nitems, per;        // (tile, plane) items and items per tile block
-    int zlo, zhi;               // planes whose tiled cells use the interior row
-    int cxf, cyf;               // (x, y) class of the tiled cells
-    // edge cells, per plane: rows y < oy and y >= oy+ly (all x), then for
-    // y in [oy, oy+ly) the cells x < ox and x >= ox+lx
-    int ring, nly, nhy, nlx, nhx;
-    int64_t nedge, eper;
+    int ntx, nty, nblocks;
+    int64_t nitems, per;   // (tile, plane) items and items per block
+    int lo[3], hi[3];      // interior range of each axis (extent 1: its cell; extent 2: empty)
 };
 
 template <typename TV, bool FUSE>
@@ -254,17 +248,18 @@ __device__ __forceinline__ TV row_lds(const TC *row, const TV *const (&P3)[3], i
     return acc;
 }
 
-// Uniform 27-slot row against the LDS planes (boundary planes only); rolled
-// over the nine (oz, oy) lines to keep the register count of the kernel low.
-template <typename TC, typename TV>
-__device__ __forceinline__ TV row27_lds(scalar_ptr<TC> row, const TV *const (&P3)[3], int base) {
+// 27-slot class row (slot = (oz+1)*9 + (oy+1)*3 + (ox+1)) against the LDS
+// planes, one oz plane at a time so these rare rows do not raise the kernel's
+// register count.  R is a scalar (wave-uniform) or per-lane row pointer.
+template <typename R, typename TV>
+__device__ __forceinline__ TV row27_lds(R row, const TV *const (&P3)[3], int base) {
     TV acc = TV(0);
 #pragma unroll 1
-    for(int l = 0; l < 9; l++) {
-        const TV *P = P3[l / 3] + base + (l % 3 - 1) * kW;
-        acc += (TV)row[3 * l] * P[-1];
-        acc += (TV)row[3 * l + 1] * P[0];
-        acc += (TV)row[3 * l + 2] * P[1];
+    for(int oz = 0; oz < 3; oz++) {
+        const TV *P = (oz == 0 ? P3[0] : (oz == 1 ? P3[1] : P3[2])) + base;
+        const R r = row + 9 * oz;
+#pragma unroll 3
+        for(int j = 0; j < 9; j++) acc += (TV)r[j] * P[(j / 3 - 1) * kW + (j % 3 - 1)];
     }
     return acc;
 }
@@ -275,7 +270,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
                                              const TV *__restrict__ rz, TV beta, TV *__restrict__ p_out, const Epi &epi,
                                              double (&v)[Epi::NR]) {
     __shared__ TV pl[3 * kPL];
-    const int x0 = g.ox + (t % g.ntx) * kTX, y0 = g.oy + (t / g.ntx) * kTY;
+    const int x0 = (t % g.ntx) * kTX, y0 = (t / g.ntx) * kTY;
     const int tid = threadIdx.x;
     const int64_t sz = (int64_t)op.nx * op.ny;
     int goff[kNLD];
@@ -288,8 +283,10 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
         gok[j] = idx < kPL && gx >= 0 && gx < op.nx && gy >= 0 && gy < op.ny;
         goff[j] = gok[j] ? gx + op.nx * gy : 0;
     }
-    TV reg[kNLD];
-    auto fetch = [&](int z) {
+    // two register sets keep the loads of planes z+2 and z+3 in flight while
+    // plane z is computed
+    TV ra[kNLD], rb[kNLD];
+    auto fetch = [&](TV(&reg)[kNLD], int z) {
         const bool zin = z >= 0 && z < op.nz;
         const int64_t zo = sz * (zin ? z : 0);
 #pragma unroll
@@ -298,75 +295,108 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
             reg[j] = (zin && gok[j]) ? val : TV(0);
         }
     };
-    auto put = [&](int slot) {
+    auto put = [&](const TV(&reg)[kNLD], int slot) {
 #pragma unroll
         for(int j = 0; j < kNLD; j++)
             if(tid + j * kBlock < kPL) pl[slot * kPL + tid + j * kBlock] = reg[j];
     };
-    fetch(z0 - 1);
-    put((z0 + 2) % 3);
-    fetch(z0);
-    put(z0 % 3);
-    fetch(z0 + 1);
     const int tx = tid % kTX, ty = tid / kTX;
     const int gx = x0 + tx;
-    const bool xin = gx < g.ox + g.lx;
-    for(int z = z0; z < z1; z++) {
-        put((z + 1) % 3);
-        if(z + 1 < z1) fetch(z + 2);
-        __syncthreads();
+    // class mode: lanes on an x face are evaluated by the face pass below
+    const bool xf = !CLS || (gx >= g.lo[0] && gx <= g.hi[0]);
+    // face pass: lanes 0..15 of wave 0 take the tile's x-face cells (row l & 7,
+    // left face for l < 8 when the tile holds x = 0, right face otherwise)
+    const int fx = tid < 8 ? 0 : op.nx - 1;
+    const bool face_lane = CLS && tid < 16 && fx >= x0 && fx < x0 + kTX && (fx < g.lo[0] || fx > g.hi[0]) && y0 + (tid & 7) < op.ny;
+    auto finish = [&](int64_t c, TV acc, TV xc, double dg) {
+        if constexpr(FUSE) p_out[c] = xc;
+        epi(c, acc, xc, dg, v);
+    };
+    // plane z from LDS (planes z-1, z, z+1 staged)
+    auto compute = [&](int z) {
         const TV *const P3[3] = {pl + ((z + 2) % 3) * kPL, pl + (z % 3) * kPL, pl + ((z + 1) % 3) * kPL};
         TC crow[PAT];
         bool zf = true;
-        scalar_ptr<TC> brow = nullptr;
         if constexpr(CLS) {
-            zf = z >= g.zlo && z <= g.zhi;
+            zf = z >= g.lo[2] && z <= g.hi[2];
             if(zf) {
                 const scalar_ptr<TC> row = as_scalar(op.table_int + z * PAT);
 #pragma unroll
                 for(int k = 0; k < PAT; k++) crow[k] = row[k];
-            } else {
-                brow = as_scalar(op.table + ((z * 3 + g.cyf) * 3 + g.cxf) * 27);
             }
         }
-#pragma unroll
+        // per-cell rows: one row at a time (its PAT coefficient loads are in flight)
+#pragma unroll CLS ? kRPT : 1
         for(int q = 0; q < kRPT; q++) {
             const int rr = ty + q * (kBlock / kTX);
             const int gy = y0 + rr;
-            if(!xin || gy >= g.oy + g.ly) continue;
+            if(gx >= op.nx || gy >= op.ny) continue;
             const int64_t c = gx + (int64_t)op.nx * gy + sz * z;
             const int base = (rr + 1) * kW + tx + 1;
             TV acc;
             double dg = 0.0;  // the row's diagonal (Jacobi epilogues only)
             if constexpr(!CLS) {
-                const TC *col = op.percell + c;
+                // slot arrays start at uniform addresses; a 32-bit cell offset
+                // lets every load use a scalar base (callers keep n <= 2^31)
+                const uint32_t cu = (uint32_t)c;
                 acc = TV(0);
 #pragma unroll
                 for(int k = 0; k < PAT; k++)
-                    acc += (TV)col[(int64_t)k * op.n] *
+                    acc += (TV)(op.percell + (int64_t)k * op.n)[cu] *
                            P3[RowPattern<PAT>::o[k][2] + 1][base + RowPattern<PAT>::o[k][1] * kW + RowPattern<PAT>::o[k][0]];
-                if constexpr(Epi::kDiag) dg = (double)col[0];
-            } else if(zf) {
-                acc = row_lds<PAT>(crow, P3, base);
-                if constexpr(Epi::kDiag) dg = (double)crow[0];
+                if constexpr(Epi::kDiag) dg = (double)op.percell[cu];
             } else {
-                acc = row27_lds<TC>(brow, P3, base);
-                if constexpr(Epi::kDiag) dg = (double)brow[13];
+                if(zf && gy >= g.lo[1] && gy <= g.hi[1]) {
+                    acc = row_lds<PAT>(crow, P3, base);
+                    if constexpr(Epi::kDiag) dg = (double)crow[0];
+                } else {
+                    // y face or boundary plane: the row of class (z, y class, interior x)
+                    const int cyc = gy == 0 ? 0 : (gy == op.ny - 1 ? 2 : 1), cxc = op.nx == 1 ? 0 : 1;
+                    const scalar_ptr<TC> row = as_scalar(op.table + ((z * 3 + cyc) * 3 + cxc) * 27);
+                    acc = row27_lds(row, P3, base);
+                    if constexpr(Epi::kDiag) dg = (double)row[13];
+                }
+                if(!xf) continue;
             }
-            const TV xc = P3[1][base];
-            if constexpr(FUSE) p_out[c] = xc;
-            epi(c, acc, xc, dg, v);
+            finish(c, acc, P3[1][base], dg);
         }
+        if constexpr(CLS) {
+            if(face_lane) {
+                const int rr = tid & 7, gy = y0 + rr;
+                const int64_t c = fx + (int64_t)op.nx * gy + sz * z;
+                const int base = (rr + 1) * kW + (fx - x0) + 1;
+                const int cyc = gy == 0 ? 0 : (gy == op.ny - 1 ? 2 : 1), cxc = fx == 0 ? 0 : 2;
+                const TC *row = op.table + ((z * 3 + cyc) * 3 + cxc) * 27;
+                finish(c, row27_lds(row, P3, base), P3[1][base], (double)row[13]);
+            }
+        }
+    };
+    fetch(ra, z0 - 1);
+    put(ra, (z0 + 2) % 3);
+    fetch(ra, z0);
+    put(ra, z0 % 3);
+    fetch(ra, z0 + 1);
+    if(z0 + 2 <= z1) fetch(rb, z0 + 2);
+    for(int z = z0; z < z1; z += 2) {
+        put(ra, (z + 1) % 3);  // plane z+1
+        if(z + 3 <= z1) fetch(ra, z + 3);
+        __syncthreads();
+        compute(z);
+        __syncthreads();
+        if(z + 1 >= z1) break;
+        put(rb, (z + 2) % 3);  // plane z+2
+        if(z + 4 <= z1) fetch(rb, z + 4);
+        __syncthreads();
+        compute(z + 1);
         __syncthreads();
     }
 }
 
 template <typename TV, typename TC, bool CLS, int PAT, bool FUSE, class Epi>
-__device__ __forceinline__ void tile_sweep(const OpView<TC> &op, const TileGeom &g, int b, int nb, const TV *__restrict__ x,
-                                           const TV *__restrict__ rz, TV beta, TV *__restrict__ p_out, const Epi &epi,
-                                           double (&v)[Epi::NR]) {
+__device__ __forceinline__ void tile_sweep(const OpView<TC> &op, const TileGeom &g, const TV *__restrict__ x, const TV *__restrict__ rz,
+                                           TV beta, TV *__restrict__ p_out, const Epi &epi, double (&v)[Epi::NR]) {
     // runs of one XCD's blocks are adjacent (shared halos stay in its L2)
-    const int L = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int L = xcd_chunk(blockIdx.x, gridDim.x);
     int64_t w = (int64_t)L * g.per;
     const int64_t end = w + g.per < g.nitems ? w + g.per : g.nitems;
     while(w < end) {
@@ -377,46 +407,13 @@ __device__ __forceinline__ void tile_sweep(const OpView<TC> &op, const TileGeom 
     }
 }
 
-// Edge cells (class mode): class rows and neighbour values from global memory.
-template <typename TV, typename TC, bool FUSE, class Epi>
-__device__ __forceinline__ void edge_sweep(const OpView<TC> &op, const TileGeom &g, int b, const TV *__restrict__ x,
-                                           const TV *__restrict__ rz, TV beta, TV *__restrict__ p_out, const Epi &epi,
-                                           double (&v)[Epi::NR]) {
-    const int64_t beg = (int64_t)b * g.eper, end = beg + g.eper < g.nedge ? beg + g.eper : g.nedge;
-    const int64_t sz = (int64_t)op.nx * op.ny;
-    const int wlo = g.nly * op.nx, whi = (g.nly + g.nhy) * op.nx, side = g.nlx + g.nhx;
-    for(int64_t i = beg + threadIdx.x; i < end; i += kBlock) {
-        const int zi = (int)(i / g.ring), r = (int)(i % g.ring);
-        int xi, yi;
-        if(r < wlo) {
-            yi = r / op.nx;
-            xi = r % op.nx;
-        } else if(r < whi) {
-            yi = g.oy + g.ly + (r - wlo) / op.nx;
-            xi = (r - wlo) % op.nx;
-        } else {
-            const int j = (r - whi) % side;
-            yi = g.oy + (r - whi) / side;
-            xi = j < g.nlx ? j : g.ox + g.lx + (j - g.nlx);
-        }
-        const int64_t c = xi + (int64_t)op.nx * yi + sz * zi;
-        const TC *row = op.table + ((zi * 3 + pos_cls(yi, op.ny)) * 3 + pos_cls(xi, op.nx)) * 27;
-        TV acc = TV(0);
-#pragma unroll 1
-        for(int l = 0; l < 9; l++) {
-            const int zz = zi + l / 3 - 1, yy = yi + l % 3 - 1;
-            const bool ok = zz >= 0 && zz < op.nz && yy >= 0 && yy < op.ny;
-            const int64_t cl = c + (int64_t)(l / 3 - 1) * sz + (int64_t)(l % 3 - 1) * op.nx;
-#pragma unroll
-            for(int ox = -1; ox <= 1; ox++) {
-                const bool in = ok && xi + ox >= 0 && xi + ox < op.nx;
-                const TV val = staged<TV, FUSE>(x, rz, beta, in ? cl + ox : c);
-                acc += (TV)row[3 * l + ox + 1] * (in ? val : TV(0));
-            }
-        }
-        const TV xc = staged<TV, FUSE>(x, rz, beta, c);
-        if constexpr(FUSE) p_out[c] = xc;
-        epi(c, acc, xc, (double)row[13], v);
+// Calibration (mode -2): every block records when it started and stays for
+// `spin` ticks of the 100 MHz wall clock, so the host can count how many
+// blocks of this kernel are resident at once.
+__device__ __forceinline__ void residency_probe(int spin, double *out) {
+    const unsigned long long t0 = wall_clock64();
+    if(threadIdx.x == 0) out[blockIdx.x] = (double)t0;
+    while(wall_clock64() - t0 < (unsigned long long)spin) {
     }
 }
 
@@ -425,19 +422,17 @@ template <typename TV, typename TC, bool CLS, int PAT, bool FUSE, class Epi>
 __global__ __launch_bounds__(kBlock) void k_stencil(OpView<TC> op, TileGeom g, const TV *__restrict__ x, const TV *__restrict__ rz,
                                                     TV *__restrict__ p_out, Epi epi, CgSetup cs, int mode, int k, const double *Pin,
                                                     int nprev, double *Pout, CgScalars *sc, CgScalars *so) {
+    if(mode == -2) {
+        residency_probe(k, Pout);
+        return;
+    }
     double betad = 0.0;
     if(mode >= 0 && !iter_prologue(mode, k, Pin, nprev, sc, so, cs, betad)) return;
     const TV beta = (TV)betad;
     double v[Epi::NR];
 #pragma unroll
     for(int j = 0; j < Epi::NR; j++) v[j] = 0.0;
-    const int b = blockIdx.x;
-    if constexpr(CLS) {
-        if(b < g.nedge_blocks) edge_sweep<TV, TC, FUSE, Epi>(op, g, b, x, rz, beta, p_out, epi, v);
-        else tile_sweep<TV, TC, CLS, PAT, FUSE, Epi>(op, g, b - g.nedge_blocks, g.nblocks - g.nedge_blocks, x, rz, beta, p_out, epi, v);
-    } else {
-        tile_sweep<TV, TC, CLS, PAT, FUSE, Epi>(op, g, b, g.nblocks, x, rz, beta, p_out, epi, v);
-    }
+    tile_sweep<TV, TC, CLS, PAT, FUSE, Epi>(op, g, x, rz, beta, p_out, epi, v);
     write_partials<Epi::NR>(v, Pout);
 }
 
@@ -684,79 +679,37 @@ template <typename TC, class F> static void with_op(const OpView<TC> &op, F &&f)
     });
 }
 
-// Resident blocks of a kernel on the current device (one full wave of blocks).
-static int resident_blocks(const void *kernel) {
-    int dev = 0, cus = 0, per_cu = 0;
-    if(hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-       hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || cus <= 0 || per_cu <= 0)
-        return 1024;
-    return cus * per_cu;
-}
-
-// Tiles over the interior xy range (class mode) or the whole box (per-cell
-// mode); (tile, plane) items split over one wave of resident blocks with at
-// least kMinRun planes per block; edge blocks get a share of the wave in
-// proportion to their cells (an edge cell costs about kEdgeCost tiled cells).
-constexpr int kMinRun = 8, kEdgeCost = 6;
+// Tiles over the whole box; (tile, plane) items split over one wave of resident
+// blocks with at least kMinRun planes per block.
+constexpr int kMinRun = 8;
 template <typename TC> static TileGeom make_geom(const OpView<TC> &op, int slots) {
     TileGeom g{};
-    const bool cls = op.table != nullptr;
-    int flo[3], fhi[3];
     const int L[3] = {op.nx, op.ny, op.nz};
     for(int a = 0; a < 3; a++) {
-        // interior range of an axis; extent 1: its single cell; extent 2: empty
-        flo[a] = L[a] == 1 ? 0 : 1;
-        fhi[a] = L[a] == 1 ? 0 : L[a] - 2;
+        g.lo[a] = L[a] == 1 ? 0 : 1;
+        g.hi[a] = L[a] == 1 ? 0 : L[a] - 2;
     }
-    g.zlo = flo[2];
-    g.zhi = fhi[2];
-    if(cls) {
-        g.ox = flo[0];
-        g.oy = flo[1];
-        g.lx = fhi[0] - flo[0] + 1 > 0 ? fhi[0] - flo[0] + 1 : 0;
-        g.ly = fhi[1] - flo[1] + 1 > 0 ? fhi[1] - flo[1] + 1 : 0;
-        g.cxf = g.lx ? pos_cls(g.ox, op.nx) : 1;
-        g.cyf = g.ly ? pos_cls(g.oy, op.ny) : 1;
-        g.nlx = g.ox;
-        g.nhx = op.nx - g.ox - g.lx;
-        g.nly = g.oy;
-        g.nhy = op.ny - g.oy - g.ly;
-        g.ring = (g.nly + g.nhy) * op.nx + g.ly * (g.nlx + g.nhx);
-        g.nedge = (int64_t)g.ring * op.nz;
-    } else {
-        g.lx = op.nx;
-        g.ly = op.ny;
-    }
-    g.ntx = (g.lx + kTX - 1) / kTX;
-    g.nty = (g.ly + kTY - 1) / kTY;
-    g.nitems = g.lx && g.ly ? (int64_t)g.ntx * g.nty * op.nz : 0;
-    int64_t ne = 0;
-    if(g.nedge > 0) {
-        ne = (int64_t)((double)slots * kEdgeCost * (double)g.nedge / (double)(kEdgeCost * g.nedge + (int64_t)g.lx * g.ly * op.nz));
-        const int64_t cap = (g.nedge + kBlock - 1) / kBlock;
-        if(ne > cap) ne = cap;
-        if(ne < 1) ne = 1;
-        if(ne > 1024) ne = 1024;
-    }
-    int64_t nt = (g.nitems + kMinRun - 1) / kMinRun;
-    if(nt > slots - ne) nt = slots - ne;
-    if(nt > kMaxBlocks - ne) nt = kMaxBlocks - ne;
-    if(nt >= 16) nt = nt / 8 * 8;  // whole rounds of the XCD dispatch order
-    if(nt < 1) nt = g.nitems || !ne ? 1 : 0;
-    g.nedge_blocks = (int)ne;
-    g.nblocks = (int)(ne + nt);
-    g.per = nt ? (g.nitems + nt - 1) / nt : 0;
-    g.eper = ne ? (g.nedge + ne - 1) / ne : 0;
+    g.ntx = (op.nx + kTX - 1) / kTX;
+    g.nty = (op.ny + kTY - 1) / kTY;
+    g.nitems = (int64_t)g.ntx * g.nty * op.nz;
+    int64_t nb = (g.nitems + kMinRun - 1) / kMinRun;
+    if(nb > slots) nb = slots;
+    if(nb > kMaxBlocks) nb = kMaxBlocks;
+    if(nb >= 16) nb = nb / 8 * 8;  // whole rounds of the XCD dispatch order
+    if(nb < 1) nb = 1;
+    g.nblocks = (int)nb;
+    g.per = (g.nitems + nb - 1) / nb;
     return g;
 }
 
-// Geometry for one kernel instance, computed once per device.
-template <auto K, typename TC> static TileGeom geom_for(const OpView<TC> &op) {
-    static int slots[64] = {};
-    int dev = 0;
-    if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    if(!slots[dev]) slots[dev] = resident_blocks(reinterpret_cast<const void *>(K));
-    return make_geom(op, slots[dev]);
+// Count of blocks that started within 10 us of the first in a calibration
+// launch (start times in 100 MHz ticks).
+static int count_resident(const double *start, int n) {
+    double t0 = start[0];
+    for(int i = 1; i < n; i++) t0 = start[i] < t0 ? start[i] : t0;
+    int c = 0;
+    for(int i = 0; i < n; i++) c += start[i] < t0 + 1000.0;
+    return c;
 }
 
 static int check_pattern(int pat) {
@@ -820,12 +773,36 @@ static int poll(CgScalars *dev, CgScalars *host, hipStream_t s) {
 }
 
 // Launch one stencil kernel instance over its balanced geometry; returns its
-// block count (the number of partial sums it leaves).
+// block count (the number of partial sums it leaves), or -1 on a HIP error.
+// On first use per device the instance is calibrated: a launch in mode -2 with
+// more blocks than fit, each staying 30 us, shows how many are resident at
+// once (the occupancy query alone over-counts kernels with many scalar
+// registers); the start times go through Pout, which holds nothing live yet.
 template <typename TV, typename TC, int C, int P, bool FUSE, class Epi>
 static int launch_stencil(const OpView<TC> &op, const TV *x, const TV *rz, TV *p_out, const Epi &epi, const CgSetup &cs, int mode,
                           int k, const double *Pin, int nprev, double *Pout, CgScalars *sc, CgScalars *so, hipStream_t s) {
     constexpr auto K = &k_stencil<TV, TC, C != 0, P, FUSE, Epi>;
-    const TileGeom g = geom_for<K>(op);
+    static int slots[64] = {};
+    int dev = 0;
+    if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if(!slots[dev]) {
+        int cus = 0, per_cu = 0;
+        MONO_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        MONO_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(K), kBlock, 0));
+        int n = cus * (per_cu + 1);
+        if(n > kMaxBlocks * kMaxRed) n = kMaxBlocks * kMaxRed;
+        if(n < 1) n = 1;
+        K<<<n, kBlock, 0, s>>>(op, TileGeom{}, x, rz, p_out, epi, cs, -2, 3000, nullptr, 0, Pout, nullptr, nullptr);
+        MONO_CHECK(hipGetLastError());
+        std::vector<double> start(n);
+        MONO_CHECK(hipMemcpyAsync(start.data(), Pout, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        MONO_CHECK(hipStreamSynchronize(s));
+        slots[dev] = count_resident(start.data(), n);
+        if(getenv("MONO_TRACE_SLOTS"))
+            fprintf(stderr, "[mono] stencil<%s,%d,%d,%d,NR=%d>: %d blocks resident (occupancy query %d x %d CUs)\n",
+                    sizeof(TV) == 8 ? "f64" : "f32", C, P, (int)FUSE, Epi::NR, slots[dev], per_cu, cus);
+    }
+    const TileGeom g = make_geom(op, slots[dev]);
     K<<<g.nblocks, kBlock, 0, s>>>(op, g, x, rz, p_out, epi, cs, mode, k, Pin, nprev, Pout, sc, so);
     return g.nblocks;
 }
@@ -846,6 +823,7 @@ int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x
             nres = launch_stencil<double, double, C, P, false>(op, x, nullptr, nullptr, EpiRes64<false>{b, r, pb[0], z}, cs, -1, 0,
                                                                nullptr, 0, P0, sc, nullptr, s);
     });
+    if(nres < 0) return -1;
     MONO_CHECK(hipGetLastError());
     const double *rz = cfg.jacobi ? z : r;
     int enq = 0;
@@ -861,6 +839,7 @@ int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x
                 nst = launch_stencil<double, double, C, P, true>(op, pin, rz, pout, EpiSpmv<double>{q}, cs, k == 0 ? 0 : 1, k, P0,
                                                                  k == 0 ? nres : grid, P1, sc, nullptr, s);
             });
+            if(nst < 0) return -1;
             // alpha from P1; x += alpha p; r -= alpha q; r.r, r.z -> P0
             k_update<double, double, double><<<grid, kBlock, 0, s>>>(op, x, pout, q, r, z, cfg.jacobi, k, P1, nst, P0, sc);
         }
@@ -897,6 +876,7 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
             nres = launch_stencil<double, double, C, P, false>(op64, x, nullptr, nullptr, EpiResOuter{b, r32, pb[0], d32}, cs, -1, 0,
                                                                nullptr, 0, P0, sc, nullptr, s);
         });
+        if(nres < 0) return -1;
         MONO_CHECK(hipGetLastError());
         int chunk = cfg.poll_chunk > 0 ? cfg.poll_chunk : chunk0;
         int enq = 0;
@@ -910,6 +890,7 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
                     nst = launch_stencil<float, float, C, P, true>(op32, pin, r32, pout, EpiSpmv<float>{q32}, cs, k == 0 ? 2 : 1, k, P0,
                                                                    k == 0 ? nres : grid, P1, sc_in, sc, s);
                 });
+                if(nst < 0) return -1;
                 k_update<float, float, float><<<grid, kBlock, 0, s>>>(op32, d32, pout, q32, r32, nullptr, 0, k, P1, nst, P0, sc_in);
             }
             MONO_CHECK(hipGetLastError());

@@ -1,0 +1,112 @@
+// Activation times and action-potential durations on the device.
+//
+// Restates save_with_activation_times (save_mesh.c:730-913) per cell, on the
+// device, at each save call (every print_rate iterations, before the step):
+//   - the first call (t == 0) only records Vm;
+//   - an upward crossing of the activation threshold between the previous and
+//     the current call records an activation at the current time, unless the
+//     cell already has one less than time_threshold ms earlier;
+//   - a cell with a pending activation records APD = t - activation time when
+//     Vm falls to the APD threshold (or at once for the earlier of two pending
+//     activations).
+// The reference keeps Vm, counters and times in float (its hash values and
+// arrays are float); so does this kernel.  Times and APDs beyond max_events per
+// cell are counted but not stored (their APDs come out as NaN).
+#include "mono_common.h"
+#include "observer.h"
+
+#include <vector>
+
+namespace mono {
+
+__global__ __launch_bounds__(256) void k_observe(int64_t n, const double *__restrict__ v, const uint8_t *__restrict__ active, double t,
+                                                 float act_thr, float apd_thr, float time_thr, int kmax, float *__restrict__ last_v,
+                                                 float *__restrict__ was_active, int32_t *__restrict__ n_act, int32_t *__restrict__ n_apd,
+                                                 float *__restrict__ act, float *__restrict__ apd) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    if(active && !active[i]) return;
+    const double vi = v[i];
+    if(t == 0.0) {
+        last_v[i] = (float)vi;
+        return;
+    }
+    const float lv = last_v[i];
+    int na = n_act[i];
+    float wa = was_active[i];
+    if(lv < act_thr && vi >= (double)act_thr) {
+        bool rec = na == 0;
+        if(!rec) {
+            const double last = na - 1 < kmax ? (double)act[(int64_t)(na - 1) * n + i] : (double)NAN;
+            rec = !(t - last <= (double)time_thr);  // a lost time does not block a new activation
+        }
+        if(rec) {
+            if(na < kmax) act[(int64_t)na * n + i] = (float)t;
+            na++;
+            wa += 1.0f;
+        }
+    }
+    if(wa != 0.0f && (vi <= (double)apd_thr || wa == 2.0f)) {
+        const int tmp = (int)wa;
+        const int k = na - tmp;
+        const double last = k >= 0 && k < kmax ? (double)act[(int64_t)k * n + i] : (double)NAN;
+        const int np = n_apd[i];
+        if(np < kmax) apd[(int64_t)np * n + i] = (float)(t - last);
+        n_apd[i] = np + 1;
+        wa = (float)(tmp - 1);
+    }
+    last_v[i] = (float)vi;
+    n_act[i] = na;
+    was_active[i] = wa;
+}
+
+int Observer::init(int64_t n_, float a, float p, float tt, int kmax, hipStream_t s) {
+    if(release()) return -1;
+    n = n_;
+    act_thr = a;
+    apd_thr = p;
+    time_thr = tt;
+    max_events = kmax;
+    const size_t nn = (size_t)(n > 0 ? n : 1), kk = (size_t)(kmax > 0 ? kmax : 1);
+    MONO_CHECK(hipMalloc(&last_v, sizeof(float) * nn));
+    MONO_CHECK(hipMalloc(&was_active, sizeof(float) * nn));
+    MONO_CHECK(hipMalloc(&n_act, sizeof(int32_t) * nn));
+    MONO_CHECK(hipMalloc(&n_apd, sizeof(int32_t) * nn));
+    MONO_CHECK(hipMalloc(&act, sizeof(float) * nn * kk));
+    MONO_CHECK(hipMalloc(&apd, sizeof(float) * nn * kk));
+    // reference defaults: last Vm -100, no activations
+    const float m100 = -100.0f;
+    std::vector<float> h(nn, m100);
+    MONO_CHECK(hipMemcpyAsync(last_v, h.data(), sizeof(float) * nn, hipMemcpyHostToDevice, s));
+    MONO_CHECK(hipMemsetAsync(was_active, 0, sizeof(float) * nn, s));
+    MONO_CHECK(hipMemsetAsync(n_act, 0, sizeof(int32_t) * nn, s));
+    MONO_CHECK(hipMemsetAsync(n_apd, 0, sizeof(int32_t) * nn, s));
+    MONO_CHECK(hipMemsetAsync(act, 0xff, sizeof(float) * nn * kk, s));  // NaN
+    MONO_CHECK(hipMemsetAsync(apd, 0xff, sizeof(float) * nn * kk, s));
+    MONO_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int Observer::release() {
+    void *bufs[] = {last_v, was_active, n_act, n_apd, act, apd};
+    for(void *p : bufs)
+        if(p) MONO_CHECK(hipFree(p));
+    last_v = was_active = act = apd = nullptr;
+    n_act = n_apd = nullptr;
+    max_events = 0;
+    return 0;
+}
+
+int Observer::observe(const double *v, const uint8_t *active, double t, hipStream_t s) {
+    if(!last_v) {
+        mono_set_error("activation observer not set up");
+        return -1;
+    }
+    if(n == 0) return 0;
+    k_observe<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, v, active, t, act_thr, apd_thr, time_thr, max_events, last_v, was_active,
+                                                          n_act, n_apd, act, apd);
+    MONO_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace mono

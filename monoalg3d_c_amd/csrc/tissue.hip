@@ -15,6 +15,7 @@
 
 #include "../../include/monoalg_hip.h"
 #include "cg.h"
+#include "observer.h"
 #include "ionic.h"
 #include "mono_common.h"
 #include "operator.h"
@@ -44,6 +45,7 @@ struct mono_tissue {
     Operator op;
     CgWorkspace cg;
     CgConfig cgcfg;
+    Observer obs;
     hipStream_t stream = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     int timing = 0;
@@ -268,6 +270,7 @@ int mono_tissue_destroy(mono_tissue *t) {
     if(t->stream && hipStreamSynchronize(t->stream) != hipSuccess) rc = -1;
     if(t->cg.release()) rc = -1;
     if(t->op.release()) rc = -1;
+    if(t->obs.release()) rc = -1;
     void *bufs[] = {t->sv, t->v, t->b, t->active, t->g, t->tensor};
     for(void *p : bufs)
         if(p && hipFree(p) != hipSuccess) rc = -1;
@@ -276,6 +279,37 @@ int mono_tissue_destroy(mono_tissue *t) {
     if(t->stream && hipStreamDestroy(t->stream) != hipSuccess) rc = -1;
     delete t;
     return rc;
+}
+
+int mono_tissue_observe_setup(mono_tissue *t, float activation_threshold, float apd_threshold, float time_threshold,
+                              int max_events) {
+    if(!t || max_events < 1 || max_events > 1024) {
+        mono_set_error("mono_tissue_observe_setup: bad arguments");
+        return -1;
+    }
+    return t->obs.init(t->n, activation_threshold, apd_threshold, time_threshold, max_events, t->stream);
+}
+
+int mono_tissue_observe(mono_tissue *t) {
+    if(!t) {
+        mono_set_error("null tissue");
+        return -1;
+    }
+    return t->obs.observe(t->v, t->active, t->cur_time, t->stream);
+}
+
+int mono_tissue_get_activations(mono_tissue *t, int32_t *n_act, float *act, int32_t *n_apd, float *apd) {
+    if(!t || !t->obs.last_v) {
+        mono_set_error("activation observer not set up");
+        return -1;
+    }
+    const size_t n = (size_t)t->n, k = (size_t)t->obs.max_events;
+    if(n_act) MONO_CHECK(hipMemcpyAsync(n_act, t->obs.n_act, sizeof(int32_t) * n, hipMemcpyDeviceToHost, t->stream));
+    if(act) MONO_CHECK(hipMemcpyAsync(act, t->obs.act, sizeof(float) * n * k, hipMemcpyDeviceToHost, t->stream));
+    if(n_apd) MONO_CHECK(hipMemcpyAsync(n_apd, t->obs.n_apd, sizeof(int32_t) * n, hipMemcpyDeviceToHost, t->stream));
+    if(apd) MONO_CHECK(hipMemcpyAsync(apd, t->obs.apd, sizeof(float) * n * k, hipMemcpyDeviceToHost, t->stream));
+    MONO_CHECK(hipStreamSynchronize(t->stream));
+    return 0;
 }
 
 int64_t mono_tissue_num_cells(const mono_tissue *t) { return t ? t->n : -1; }
@@ -562,6 +596,10 @@ int mono_cg_stencil(int nx, int ny, int nz, const double *coef27, const uint8_t 
                     double rel_tol, int max_its, int jacobi, int precision, int *iterations, double *error) {
     if(nx <= 0 || ny <= 0 || nz <= 0 || !coef27 || !b || !x) {
         mono_set_error("mono_cg_stencil: bad arguments");
+        return -1;
+    }
+    if((int64_t)nx * ny * nz > (int64_t)1 << 31) {
+        mono_set_error("mono_cg_stencil: too many cells: %lld", (long long)nx * ny * nz);
         return -1;
     }
     if(mono_device_count() < 1) {

@@ -210,3 +210,38 @@ class Tissue:
     @property
     def total_cg_its(self) -> int:
         return self._t.total_cg_its
+
+
+class Observer:
+    """Activation times and APDs as save_with_activation_times keeps them
+    (save_mesh.c:754-913), vectorised over cells: Vm, counters and times in
+    float32 as in the reference's hash values and arrays."""
+
+    def __init__(self, n, act_thr=-30.0, apd_thr=-83.0, time_thr=10.0):
+        self.act_thr, self.apd_thr, self.time_thr = np.float32(act_thr), np.float32(apd_thr), np.float32(time_thr)
+        self.last_v = np.full(n, -100.0, np.float32)   # hmdefault(last_time_v, -100.0)
+        self.was_active = np.zeros(n, np.float32)       # cell_was_active
+        self.acts = [[] for _ in range(n)]              # activation_times (float arrays)
+        self.apds = [[] for _ in range(n)]
+
+    def observe(self, v, t, active=None):
+        """One save call at time t (double) on Vm v (double)."""
+        v = np.asarray(v, np.float64)
+        cells = np.arange(len(v)) if active is None else np.nonzero(active)[0]
+        if t == 0.0:                                     # :836-837
+            self.last_v[cells] = v[cells].astype(np.float32)
+            return
+        thr = np.float64(self.act_thr)
+        cross = (self.last_v[cells] < self.act_thr) & (v[cells] >= thr)   # :839
+        for c in cells[cross]:
+            a = self.acts[c]
+            if not a or t - np.float64(a[-1]) > np.float64(self.time_thr):  # :840-857
+                a.append(np.float32(t))
+                self.was_active[c] += np.float32(1.0)
+        for c in cells[self.was_active[cells] != 0.0]:   # :861-874
+            if v[c] <= np.float64(self.apd_thr) or self.was_active[c] == 2.0:
+                tmp = int(self.was_active[c])
+                last = np.float64(self.acts[c][len(self.acts[c]) - tmp])
+                self.apds[c].append(np.float32(t - last))
+                self.was_active[c] = np.float32(tmp - 1)
+        self.last_v[cells] = v[cells].astype(np.float32)   # :875

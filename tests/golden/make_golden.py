@@ -263,6 +263,63 @@ def tissue(tmp, name, shape, model, T, every, aniso=False, fibrosis=0.0, keep_ti
                                   f"-30 mV interpolated between saved samples")
 
 
+def read_acm(path, shape):
+    """The reference's activation_info .acm file (save_mesh.c:754-913): one line
+    per active cell, 'cx,cy,cz,hx,hy,hz n [ t1 .. tn ] [ apd1 .. ]'."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    lines = open(path).read().splitlines()
+    last = int(lines[0])
+    nact = np.zeros(n, np.int32)
+    acts, apds = [[] for _ in range(n)], [[] for _ in range(n)]
+    for ln in lines[1:]:
+        geo, rest = ln.split(" ", 1)
+        cx, cy, cz = (float(v) for v in geo.split(",")[:3])
+        c = int(np.floor(cx / H)) + nx * (int(np.floor(cy / H)) + ny * int(np.floor(cz / H)))
+        k = rest.index("[")
+        nact[c] = int(rest[:k])
+        a, b = rest[k:].split("]", 1)
+        acts[c] = [float(v) for v in a.strip("[ ").split()]
+        apds[c] = [float(v) for v in b.strip().strip("[]").split()]
+    m = max(1, max(len(a) for a in acts), max(len(a) for a in apds))
+    act = np.full((n, m), np.nan)
+    apd = np.full((n, m), np.nan)
+    for c in range(n):
+        act[c, :len(acts[c])] = acts[c]
+        apd[c, :len(apds[c])] = apds[c]
+    return last, nact, act, apd
+
+
+def activation(tmp):
+    """G7: activation times and APDs from the reference's save_with_activation_times
+    (save_mesh.c:730-913) on a paced TT2006 slab: 12x12x2 cells, isotropic,
+    stimuli at t = 0 and 400 ms, 700 ms, sampled every 10 steps."""
+    shape = (12, 12, 2)
+    nx, ny, nz = shape
+    out_dir = os.path.join(tmp, "act")
+    os.makedirs(out_dir, exist_ok=True)
+    domain = (f"name=Cuboid Mesh\nstart_dx={H}\nstart_dy={H}\nstart_dz={H}\nside_length_x={nx * H}\n"
+              f"side_length_y={ny * H}\nside_length_z={nz * H}\nmain_function=initialize_grid_with_cuboid_mesh")
+    asm = "sigma_x=0.00013\nsigma_y=0.00013\nsigma_z=0.00013\nmain_function=homogeneous_sigma_assembly_matrix"
+    ini = ini_text(out_dir, 700.0, MODELS["tt2006"][0], domain, asm, 10)
+    ini = ini.replace("main_function=save_as_text_or_binary\nfile_prefix=V\nbinary=true",
+                      "main_function=save_with_activation_times\ninit_function=init_save_with_activation_times\n"
+                      "end_function=end_save_with_activation_times\nactivation_threshold=-30.0\napd_threshold=-83.0\n"
+                      "time_threshold=10.0")
+    ini = ini.replace("x_limit = 500.0", "x_limit = 500.0\nperiod = 400.0")
+    path = os.path.join(out_dir, "run.ini")
+    with open(path, "w") as fh:
+        fh.write(ini)
+    run([os.path.join(REF, "bin", "MonoAlg3D"), "-c", path])
+    last, nact, act, apd = read_acm(os.path.join(out_dir, "activation_info_it_0.acm"), shape)
+    np.savez_compressed(os.path.join(HERE, "activation_tt2006_12x12x2.npz"), shape=np.array(shape), print_rate=10,
+                        last_call=last, n_act=nact, act=act, apd=apd)
+    PROV["activation_tt2006_12x12x2.npz"] = (
+        "reference MonoAlg3D save_with_activation_times (activation -30 mV, APD -83 mV, time threshold 10 ms, "
+        "print_rate 10) on a 12x12x2 TT2006 isotropic 0.13 S/m cuboid, stimulus -50 on x<500 um for 2 ms every "
+        "400 ms, 700 ms; activation times and APDs per cell from activation_info_it_0.acm")
+
+
 def read_octave(path):
     """Octave text matrix (sparse: 'row col value' 1-based; dense: one value per line)."""
     lines = open(path).read().splitlines()
@@ -306,6 +363,8 @@ def main():
             tissue(tmp, "fibrotic_ord_24x24x6", (24, 24, 6), "ord2011", 10.0, 5, aniso=True, fibrosis=0.1, keep_times=(5, 10))
         if "solvers" in groups:
             solvers(tmp)
+        if "activation" in groups:
+            activation(tmp)
         if "config1" in groups:
             tissue(tmp, "config1_tt2006_64x64x16", (64, 64, 16), "tt2006", 50.0, 10, keep_times=(20,))
     finally:

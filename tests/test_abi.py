@@ -1,0 +1,164 @@
+"""The C ABI of libmonoalg_hip.so, checked without a GPU.
+
+The library must load, export every function include/monoalg_hip.h declares,
+agree with the Python binding on struct layouts (compiled from the header with
+gcc), answer the host-only queries (model sizes and initial states, which must
+equal the reference's, cf. ten_tusscher_2006.h:7 and the models'
+set_model_initial_conditions_cpu) and refuse device work with an error code
+and message instead of falling back to anything.
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "monoalg_hip.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mono_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_matches_binding():
+    from monoalg3d_c_amd import _lib
+    assert sorted(_lib.EXPORTED) == _declared()
+
+
+def test_library_exports_every_declared_symbol():
+    from monoalg3d_c_amd import _lib
+    L = _lib.lib()
+    missing = [s for s in _declared() if not hasattr(L, s)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert set(_declared()) <= exported
+
+
+def test_struct_layout_matches_header(tmp_path):
+    """sizeof/offsetof of the ABI structs as gcc lays them out from the header."""
+    from monoalg3d_c_amd._lib import Stats, TissueDesc
+    fields = {"mono_tissue_desc": TissueDesc, "mono_stats": Stats}
+    src = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in fields.items():
+        src.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            src.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    src.append("return 0; }")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", str(c), "-o", str(exe)], check=True)
+    got = {}
+    for ln in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        s, f, v = ln.split()
+        got[(s, f)] = int(v)
+    for cname, py in fields.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_model_queries_match_oracle(oracle):
+    import monoalg3d_c_amd as m
+    for name, mid in m.MODELS.items():
+        assert m.model_neq(mid) == oracle.neq(oracle.MODEL_IDS[name])
+        assert m.model_initial_v(mid) == oracle.initial_v(oracle.MODEL_IDS[name])
+        np.testing.assert_array_equal(m.model_initial_state(mid), oracle.model_init(oracle.MODEL_IDS[name], 1)[0])
+    from monoalg3d_c_amd import _lib
+    assert _lib.lib().mono_model_neq(7) == -1
+    assert _lib.lib().mono_model_initial_state(9, None) == -1
+    assert b"model" in _lib.lib().mono_last_error()
+
+
+def test_desc_defaults():
+    import monoalg3d_c_amd as m
+    d, keep = m.make_desc((4, 3, 2))
+    assert (d.nx, d.ny, d.nz) == (4, 3, 2)
+    assert d.dt_pde == 0.02 and d.dt_ode == 0.02
+    assert d.beta == 0.14 and d.cm == 1.0
+    assert d.cg_tol == 1e-16 and d.cg_max_its == 200
+    assert (d.stim_start, d.stim_duration, d.stim_current, d.stim_x_limit) == (0.0, 2.0, -50.0, 500.0)
+
+
+def test_alpha_matches_oracle(oracle):
+    import monoalg3d_c_amd as m
+    assert m.alpha() == oracle.alpha(0.14, 1.0, 0.02, 100.0, 100.0, 100.0)
+
+
+def _no_device():
+    from monoalg3d_c_amd import _lib
+    return _lib.lib().mono_device_count() == 0
+
+
+@pytest.mark.skipif("not _no_device()")
+def test_device_entry_points_fail_loudly_without_gpu():
+    """No HIP device: every compute entry point returns -2 and says why."""
+    import monoalg3d_c_amd as m
+    from monoalg3d_c_amd import _lib
+    L = _lib.lib()
+    d, keep = m.make_desc((4, 4, 2), tensor=m.iso_tensor(0.00013))
+    t = C.c_void_p()
+    assert L.mono_tissue_create(C.byref(d), C.byref(t)) == -2
+    assert b"no HIP device" in L.mono_last_error()
+    sv = np.zeros((2, 19))
+    assert L.mono_ionic_run(0, 0, sv.ctypes.data, 2, None, 0.02, 1) == -2
+    coef = np.zeros(27 * 32)
+    a = C.c_double()
+    assert L.mono_assemble(C.byref(d), coef.ctypes.data, C.byref(a)) == -2
+    with pytest.raises(m.MonoError, match="no HIP device"):
+        m.Tissue((4, 4, 2), tensor=m.iso_tensor(0.00013))
+
+
+def test_product_path_does_not_import_the_oracle():
+    """The package and its sources never reach for the oracle or the reference."""
+    pkg = os.path.join(ROOT, "monoalg3d_c_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".h", ".cpp", "Makefile")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "pyoracle" not in text and "liboracle" not in text and "/root/reference" not in text, f
+
+
+def _plugin_ini(tmp_path, shape, T, every):
+    """A reference .ini (tests/golden/make_golden.py's) with the ionic model and
+    the linear solver switched to the plugins over libmonoalg_hip.so."""
+    sys_path = os.path.join(ROOT, "tests", "golden")
+    import sys
+    if sys_path not in sys.path:
+        sys.path.insert(0, sys_path)
+    from make_golden import H, ini_text
+    nx, ny, nz = shape
+    plug = os.path.join(ROOT, "monoalg3d_c_amd", "plugins")
+    domain = (f"name=Cuboid Mesh\nstart_dx={H}\nstart_dy={H}\nstart_dz={H}\nside_length_x={nx * H}\n"
+              f"side_length_y={ny * H}\nside_length_z={nz * H}\nmain_function=initialize_grid_with_cuboid_mesh")
+    asm = "sigma_x=0.00013\nsigma_y=0.00013\nsigma_z=0.00013\nmain_function=homogeneous_sigma_assembly_matrix"
+    ini = ini_text(str(tmp_path), T, "libten_tusscher_2006.so", domain, asm, every)
+    ini = ini.replace("library_file=shared_libs/libten_tusscher_2006.so",
+                      f"library_file={os.path.join(plug, 'libmono_hip_tt2006.so')}")
+    ini = ini.replace("main_function=conjugate_gradient",
+                      f"main_function=mono_conjugate_gradient\nlibrary_file={os.path.join(plug, 'libmono_hip_cg.so')}")
+    path = tmp_path / "run.ini"
+    path.write_text(ini)
+    return str(path)
+
+
+PLUGINS = os.path.join(ROOT, "monoalg3d_c_amd", "plugins", "libmono_hip_cg.so")
+REF_EXE = os.path.join(ROOT, "oracle", "_ref", "bin", "MonoAlg3D")
+
+
+@pytest.mark.skipif("not (os.path.exists(PLUGINS) and os.path.exists(REF_EXE) and _no_device())")
+def test_plugins_bind_to_reference_core(tmp_path):
+    """The reference's own MonoAlg3D loads the plugins through its dlopen ABI:
+    the model queries and initial conditions run on the host, and the first ODE
+    solve reaches libmonoalg_hip.so, which refuses without a GPU."""
+    ini = _plugin_ini(tmp_path, (8, 8, 2), 1.0, 5)
+    r = subprocess.run([REF_EXE, "-c", ini], cwd=os.path.dirname(os.path.dirname(REF_EXE)), capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0
+    assert "libmono_hip: mono_ionic_run failed: no HIP device available" in r.stderr, r.stderr[-2000:]

@@ -1,0 +1,187 @@
+"""Full runs on the GPU against MonoAlg3D's own runs (golden fixtures).
+
+The fixtures were written by the reference binary built from its sources
+(tests/golden/make_golden.py): Vm of 12 probe cells at every saved step, Vm
+maps at fixed times and activation times (first upward crossing of -30 mV,
+linearly interpolated between saved samples).  The reference's CG stops on
+||r||^2 <= 1e-16, so FP64 runs agree to solver round-off: probe Vm within
+1e-5 mV, activation times within 1e-3 ms.  FP32 states + FP32 CG inside FP64
+defect correction must keep activation times within 0.5 ms of the reference.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+sys.path.insert(0, GOLD)
+from make_golden import lat_from  # noqa: E402  (same activation-time rule as the fixtures)
+
+DT = 0.02
+
+
+def _run(name, model, precision, aniso=False):
+    import monoalg3d_c_amd as m
+    g = np.load(os.path.join(GOLD, name + ".npz"))
+    shape = tuple(int(s) for s in g["shape"])
+    nx, ny, nz = shape
+    act = g["active"]
+    every = int(g["every"])
+    nsamp = len(g["probe_vm"])
+    if aniso:
+        t6 = np.array([m.fibre_tensor(m.normalise(f), 0.00024, 0.000045) for f in g["fibres"]])
+        kw = dict(aniso=True, tensor=t6, tensor_source=m.TENSOR_PER_CELL)
+    else:
+        kw = dict(tensor=m.iso_tensor(0.00013))
+    if not act.all():
+        kw["active"] = act
+    t = m.Tissue(shape, model=m.MODELS[model], precision=precision, cg_tol=1e-16, **kw)
+    snaps = [t.v]
+    for _ in range(nsamp - 1):
+        t.step(every)
+        snaps.append(t.v)
+    st = t.stats
+    t.close()
+    return g, np.array(snaps), st
+
+
+def _check(g, snaps, st, vm_tol, lat_tol, vm_times):
+    act = g["active"].astype(bool)
+    probes = g["probe_cells"]
+    keep = act[probes]
+    dv = np.abs(snaps[:, probes][:, keep] - g["probe_vm"][:, keep]).max()
+    lat = lat_from(snaps, int(g["every"]))
+    ref = g["lat"]
+    both = act & np.isfinite(ref)
+    assert np.array_equal(np.isfinite(lat[act]), np.isfinite(ref[act])), "different cells activated"
+    dlat = np.abs(lat[both] - ref[both]).max() if both.any() else 0.0
+    assert st["nan_detected"] == 0
+    for tms in vm_times:
+        k = int(round(tms / (int(g["every"]) * DT)))
+        dmap = np.abs(snaps[k][act] - g[f"vm_t{int(tms)}"][act]).max()
+        assert dmap < vm_tol, (tms, dmap)
+    assert dv < vm_tol, dv
+    assert dlat <= lat_tol, dlat
+    return dv, dlat
+
+
+def test_config1_fp64_matches_reference():
+    """64x64x16 TT2006, isotropic 0.13 S/m, planar stimulus, 50 ms (2,501 steps)."""
+    import monoalg3d_c_amd as m
+    announce("reference config1 64x64x16 tt2006 fp64 50 ms")
+    g, snaps, st = _run("tissue_config1_tt2006_64x64x16", "tt2006", m.FP64)
+    assert st["class_operator"] == 1 and st["stencil_points"] == 7
+    _check(g, snaps, st, 1e-5, 1e-3, (20,))
+
+
+def test_config1_fp32_refine_lat_within_half_ms():
+    import monoalg3d_c_amd as m
+    announce("reference config1 64x64x16 tt2006 fp32+refine 50 ms")
+    g, snaps, st = _run("tissue_config1_tt2006_64x64x16", "tt2006", m.FP32_REFINE)
+    act = g["active"].astype(bool)
+    lat = lat_from(snaps, int(g["every"]))
+    ref = g["lat"]
+    assert np.array_equal(np.isfinite(lat[act]), np.isfinite(ref[act]))
+    both = act & np.isfinite(ref)
+    assert np.abs(lat[both] - ref[both]).max() <= 0.5
+
+
+@pytest.mark.parametrize("name,model,aniso,vm_times", [
+    ("tissue_iso_tt2006_32x32x8", "tt2006", False, (10, 20)),
+    ("tissue_aniso_ord_24x24x6", "ord2011", True, (5, 10)),
+    ("tissue_fibrotic_ord_24x24x6", "ord2011", True, (5, 10)),
+])
+def test_tissue_fp64_matches_reference(name, model, aniso, vm_times):
+    import monoalg3d_c_amd as m
+    announce(f"reference {name} fp64")
+    g, snaps, st = _run(name, model, m.FP64, aniso)
+    _check(g, snaps, st, 1e-5, 1e-3, vm_times)
+
+
+@pytest.mark.parametrize("name,model,aniso", [
+    ("tissue_aniso_ord_24x24x6", "ord2011", True),
+    ("tissue_fibrotic_ord_24x24x6", "ord2011", True),
+])
+def test_tissue_fp32_refine_lat(name, model, aniso):
+    import monoalg3d_c_amd as m
+    announce(f"reference {name} fp32+refine")
+    g, snaps, st = _run(name, model, m.FP32_REFINE, aniso)
+    act = g["active"].astype(bool)
+    lat = lat_from(snaps, int(g["every"]))
+    ref = g["lat"]
+    both = act & np.isfinite(ref) & np.isfinite(lat)
+    assert both.sum() > 0.9 * (act & np.isfinite(ref)).sum()
+    assert np.abs(lat[both] - ref[both]).max() <= 0.5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_activation_observer_matches_reference(precision, tmp_path):
+    """Device activation times / APDs against the reference's .acm output
+    (save_with_activation_times): FP64 identical counts, times within one save
+    interval (0.2 ms) where Vm sits within rounding of a threshold at a sample
+    and APDs likewise; FP32+refine activation and APD times within 0.5 ms."""
+    import monoalg3d_c_amd as m
+    announce(f"activation observer 12x12x2 tt2006 700 ms {precision}")
+    g = np.load(os.path.join(GOLD, "activation_tt2006_12x12x2.npz"))
+    shape = tuple(int(s) for s in g["shape"])
+    every = int(g["print_rate"])
+    prec = m.FP64 if precision == "fp64" else m.FP32_REFINE
+    t = m.Tissue(shape, model=m.MODEL_TT2006, precision=prec, tensor=m.iso_tensor(0.00013), cg_tol=1e-16,
+                 stim=(0.0, 2.0, -50.0, 500.0, 400.0))
+    t.observe_setup(-30.0, -83.0, 10.0, 4)
+    count, time = 0, 0.0
+    while time - 700.0 <= DT:
+        if count % every == 0:
+            t.observe()
+        t.step(1)
+        count += 1
+        time += DT
+    a = t.activations()
+    t.close()
+    np.testing.assert_array_equal(a["n_act"], g["n_act"])
+    ref_act = g["act"][:, :2]
+    ref_apd = g["apd"][:, :2]
+    tol = 0.2 + 1e-4 if precision == "fp64" else 0.5
+    assert np.abs(a["act"][:, :2] - ref_act).max() <= tol
+    assert np.abs(a["apd"][:, :2] - ref_apd).max() <= tol
+    if precision == "fp64":
+        # nearly all events land on the same sample as the reference's
+        assert (a["act"][:, :2] == ref_act.astype(np.float32)).mean() > 0.99
+    m.write_acm(str(tmp_path / "activation_info_it_0.acm"), shape, (100.0, 100.0, 100.0), a["n_act"], a["act"], a["apd"])
+    sys.path.insert(0, GOLD)
+    from make_golden import read_acm
+    last, n_act, act, apd = read_acm(str(tmp_path / "activation_info_it_0.acm"), shape)
+    np.testing.assert_array_equal(n_act, a["n_act"])
+    np.testing.assert_allclose(act[:, :2], a["act"][:, :2], atol=1e-6)
+
+
+def test_reference_core_with_hip_plugins(tmp_path):
+    """The reference's own MonoAlg3D (built from its sources) with its ionic
+    model and linear solver swapped for the plugins over libmonoalg_hip.so
+    (monoalg3d_c_amd/plugins): the same run as the 32x32x8 TT2006 fixture,
+    Vm within 1e-5 mV and activation times within 1e-3 ms."""
+    import subprocess
+    from test_abi import PLUGINS, REF_EXE, _plugin_ini
+    from make_golden import read_snapshots
+    if not (os.path.exists(PLUGINS) and os.path.exists(REF_EXE)):
+        pytest.skip("reference build or plugins not built")
+    announce("reference core + HIP plugins 32x32x8 tt2006 20 ms")
+    g = np.load(os.path.join(GOLD, "tissue_iso_tt2006_32x32x8.npz"))
+    shape = tuple(int(s) for s in g["shape"])
+    every = int(g["every"])
+    ini = _plugin_ini(tmp_path, shape, 20.0, every)
+    r = subprocess.run([REF_EXE, "-c", ini], cwd=os.path.dirname(os.path.dirname(REF_EXE)), capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    snaps = read_snapshots(str(tmp_path), shape, every, int(round(20.0 / DT)) + 1)
+    probes = g["probe_cells"]
+    assert np.abs(snaps[:, probes] - g["probe_vm"]).max() < 1e-5
+    lat = lat_from(snaps, every)
+    both = np.isfinite(g["lat"])
+    assert np.array_equal(np.isfinite(lat), both)
+    assert np.abs(lat[both] - g["lat"][both]).max() <= 1e-3

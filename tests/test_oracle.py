@@ -266,3 +266,28 @@ def test_tissue_run_matches_reference(oracle, name, model, aniso):
     ref = g["probe_vm"]
     keep = act[g["probe_cells"]].astype(bool)
     np.testing.assert_allclose(got[:, keep], ref[:, keep], rtol=0, atol=1e-6)
+
+
+def test_activation_times_match_reference(oracle):
+    """The activation/APD restatement on the oracle's own run against the
+    reference's .acm output (12x12x2 TT2006, stimuli every 400 ms, 700 ms,
+    save every 10 steps): identical counts, times and APDs (float32)."""
+    g = _load("activation_tt2006_12x12x2")
+    shape = tuple(int(s) for s in g["shape"])
+    n = int(np.prod(shape))
+    every = int(g["print_rate"])
+    coef = oracle.assemble_iso(shape, H, _alpha(oracle), np.full((n, 3), 0.00013))
+    t = oracle.Tissue(shape, H, oracle.TT2006, coef, stim=(0.0, 2.0, -50.0, 500.0, 400.0))
+    obs = oracle.Observer(n)
+    count = 0
+    while t.time - 700.0 <= DT:                      # monodomain_solver.c:616
+        if count % every == 0:
+            obs.observe(t.v, t.time)
+        t.step()
+        count += 1
+    for c in range(n):
+        assert len(obs.acts[c]) == g["n_act"][c]
+        ref_a = g["act"][c][np.isfinite(g["act"][c])]
+        ref_p = g["apd"][c][np.isfinite(g["apd"][c])]
+        np.testing.assert_array_equal(np.array(obs.acts[c], np.float32), ref_a.astype(np.float32), err_msg=f"cell {c}")
+        np.testing.assert_array_equal(np.array(obs.apds[c], np.float32), ref_p.astype(np.float32), err_msg=f"cell {c}")
