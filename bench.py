@@ -92,26 +92,38 @@ def run_precision(shape, precision, steps, warmup):
 
 def lat_run(shape, precision, lat_ms):
     """Untimed: a fresh run to lat_ms with the activation observer on every step
-    (save_with_activation_times with print_rate 1); returns first activation times."""
+    (save_with_activation_times with print_rate 1, previous Vm kept in FP64);
+    returns first activation times and the first step that reported a NaN."""
     import monoalg3d_c_amd as m
     tis = m.Tissue(shape, **workload_desc(shape, precision))
-    tis.observe_setup(-30.0, -83.0, 10.0, 1)
-    for _ in range(int(round(lat_ms / 0.02)) + 1):
+    tis.observe_setup(-30.0, -83.0, 10.0, 1, vm_double=True)
+    nan_step = None
+    for k in range(int(round(lat_ms / 0.02)) + 1):
         tis.observe()
-        tis.step(1)
+        if tis.step(1) > 0 and nan_step is None:
+            nan_step = k
     a = tis.activations()
     tis.close()
     lat = a["act"][:, 0].astype(np.float64)
     lat[a["n_act"] == 0] = np.inf
-    return lat
+    return lat, nan_step
 
 
-def lat_error(lat64, lat32):
+def lat_error(r64, r32, lat_ms):
+    """Activation-time error of FP32+refine against FP64 (ms).  A cell activated
+    in one run only is a front cell at the end of the window: its error is at
+    least the time left to the window's end, which is what it is charged."""
+    (lat64, nan64), (lat32, nan32) = r64, r32
     act64, act32 = np.isfinite(lat64), np.isfinite(lat32)
     both = act64 & act32
-    return dict(activated_fp64=int(act64.sum()), activated_fp32=int(act32.sum()),
-                activated_in_one_only=int((act64 ^ act32).sum()),
-                max_abs_err_ms=float(np.abs(lat64[both] - lat32[both]).max()) if both.any() else None)
+    err = np.abs(lat64[both] - lat32[both]).max() if both.any() else 0.0
+    one = act64 ^ act32
+    if one.any():
+        t_one = np.where(act64, lat64, lat32)[one]
+        err = max(err, float((lat_ms + 0.02 - t_one).max()))
+    return dict(nan_step_fp64=nan64, nan_step_fp32=nan32, activated_fp64=int(act64.sum()),
+                activated_fp32=int(act32.sum()), activated_in_one_only=int(one.sum()),
+                max_abs_err_ms=float(err) if (both.any() or one.any()) else None)
 
 
 def measured_traffic(kernel, shape):
@@ -243,17 +255,19 @@ def main() -> int:
     if args.lat_ms > 0 and len(res) == 2:
         lat64 = lat_run(shape, m.FP64, args.lat_ms)
         lat32 = lat_run(shape, m.FP32_REFINE, args.lat_ms)
-        lat_check = dict(simulated_ms=args.lat_ms, **lat_error(lat64, lat32))
+        lat_check = dict(simulated_ms=args.lat_ms, **lat_error(lat64, lat32, args.lat_ms))
         lat_check["within_0p5_ms"] = (lat_check["max_abs_err_ms"] is not None and lat_check["max_abs_err_ms"] <= 0.5
-                                      and lat_check["activated_in_one_only"] == 0)
+                                      and lat_check["nan_step_fp32"] is None and lat_check["nan_step_fp64"] is None)
         print(f"[bench] lat check {lat_check}", file=sys.stderr, flush=True)
     if args.copy_calib_mb > 0:
         import torch
         nbytes = args.copy_calib_mb << 20
-        a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+        # an element-wise kernel (a plain copy_ would go to a DMA engine): reads
+        # nbytes, writes nbytes
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
         for _ in range(3):
-            b.copy_(a)
+            torch.mul(a, 1.0, out=b)
         torch.cuda.synchronize()
 
     # roofline of the dominant phase (by device time)
@@ -268,15 +282,16 @@ def main() -> int:
             solve=dict(achieved=solve_bytes / (r["solve_ms"] * 1e-3) / 1e9, bytes=solve_bytes, ms=r["solve_ms"]),
             step_bytes_per_cell=bytes_per_cell_step(neq, w, r["cg_its"] if key == "fp64" else r["inner_its"],
                                                     key == "fp32", r["outer"]))
+    # The dominant kernel is the ionic step: one launch per step, more device
+    # time than any one kernel of the solve (whose phase time spreads over the
+    # stencil, update and residual kernels; see profiles/).  Its launch time is
+    # the ionic phase, timed with HIP events on the library's stream.
     hk = "fp32" if mixed else "fp64"
-    dom = "ionic" if res[hk]["ionic_ms"] >= res[hk]["solve_ms"] else "solve"
-    d = rf[hk][dom]
-    kname = ("k_ionic_step<ORd2011," + ("float" if hk == "fp32" else "double") + ">") if dom == "ionic" else \
-        f"diffusion solve ({hk})"
+    d = rf[hk]["ionic"]
+    kname = "k_ionic_step<ORd2011," + ("float" if hk == "fp32" else "double") + ">"
     roofline = {"bound": "hbm", "achieved": round(d["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(d["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname, shape),
-                "kernel": kname, "launch_ms": round(res[hk][dom + "_ms"], 4),
-                "bytes_per_launch": int(d["bytes"])}
+                "kernel": kname, "launch_ms": round(res[hk]["ionic_ms"], 4), "bytes_per_launch": int(d["bytes"])}
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": METRIC,

@@ -142,9 +142,12 @@ int mono_tissue_device_ptrs(mono_tissue *t, void **v, void **sv, size_t *pitch, 
  * kept on the device.  Call mono_tissue_observe where the reference calls its
  * save function: every print_rate iterations, before mono_tissue_step, the
  * first time at t = 0.  Thresholds in mV and ms (reference defaults -30, -83,
- * 10); up to max_events activation times and APDs are kept per cell. */
+ * 10); up to max_events activation times and APDs are kept per cell.
+ * vm_double = 0 keeps the previous Vm in float as the reference does;
+ * 1 keeps it in FP64 (a sample rounding to the threshold then cannot hide an
+ * activation; used to compare runs of different precision). */
 int mono_tissue_observe_setup(mono_tissue *t, float activation_threshold, float apd_threshold, float time_threshold,
-                              int max_events);
+                              int max_events, int vm_double);
 int mono_tissue_observe(mono_tissue *t);
 /* n_act, n_apd: n counts; act, apd: max_events x n floats (row k = k-th event,
  * NaN where absent).  Any pointer may be NULL. */

@@ -177,9 +177,9 @@ class Tissue:
 
     # activation times and APDs (save_with_activation_times, save_mesh.c:730-913)
     def observe_setup(self, activation_threshold=-30.0, apd_threshold=-83.0, time_threshold=10.0,
-                      max_events=8) -> None:
+                      max_events=8, vm_double=False) -> None:
         check(lib().mono_tissue_observe_setup(self._h, activation_threshold, apd_threshold, time_threshold,
-                                              max_events), "mono_tissue_observe_setup")
+                                              max_events, int(vm_double)), "mono_tissue_observe_setup")
         self._max_events = max_events
 
     def observe(self) -> None:

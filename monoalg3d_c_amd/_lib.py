@@ -98,7 +98,7 @@ def lib() -> C.CDLL:
     L.mono_assemble.argtypes = [C.POINTER(TissueDesc), vp, dp]
     L.mono_cg_stencil.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
                                   C.c_int, C.POINTER(C.c_int), dp]
-    L.mono_tissue_observe_setup.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_int]
+    L.mono_tissue_observe_setup.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int]
     L.mono_tissue_observe.argtypes = [vp]
     L.mono_tissue_get_activations.argtypes = [vp, vp, vp, vp, vp]
     L.mono_cg_csr.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), dp]

@@ -17,21 +17,48 @@
 
 namespace mono {
 
+// FP64 path: the device's correctly rounded-or-nearly libm (OCML), so results
+// agree with the reference's glibc to rounding.  FP32 path: the hardware
+// approximations (v_exp_f32 / v_log_f32 based, a few ulp), which halve the
+// kernel's instruction count; its accuracy is bounded by the activation-time
+// tests, not by bitwise agreement.
+#ifndef MONO_FP32_FAST_MATH
+#define MONO_FP32_FAST_MATH 1
+#endif
 template <typename T> __device__ __forceinline__ T dexp(T x);
 template <> __device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
+#if MONO_FP32_FAST_MATH
+template <> __device__ __forceinline__ float dexp<float>(float x) { return __expf(x); }
+#else
 template <> __device__ __forceinline__ float dexp<float>(float x) { return expf(x); }
+#endif
 template <typename T> __device__ __forceinline__ T dexpm1(T x);
 template <> __device__ __forceinline__ double dexpm1<double>(double x) { return expm1(x); }
 template <> __device__ __forceinline__ float dexpm1<float>(float x) { return expm1f(x); }
 template <typename T> __device__ __forceinline__ T dlog(T x);
 template <> __device__ __forceinline__ double dlog<double>(double x) { return log(x); }
+#if MONO_FP32_FAST_MATH
+template <> __device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
+#else
 template <> __device__ __forceinline__ float dlog<float>(float x) { return logf(x); }
+#endif
 template <typename T> __device__ __forceinline__ T dpow(T x, T y);
 template <> __device__ __forceinline__ double dpow<double>(double x, double y) { return pow(x, y); }
+#if MONO_FP32_FAST_MATH
+template <> __device__ __forceinline__ float dpow<float>(float x, float y) { return __powf(x, y); }
+#else
 template <> __device__ __forceinline__ float dpow<float>(float x, float y) { return powf(x, y); }
+#endif
 template <typename T> __device__ __forceinline__ T dsqrt(T x);
 template <> __device__ __forceinline__ double dsqrt<double>(double x) { return sqrt(x); }
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
+
+// x / (exp(x) - 1), finite at x = 0 (limit 1).  The FP32 path evaluates the
+// Goldman-Hodgkin-Katz flux factors with it: written as the reference writes
+// them, v F^2/RT * (...) / (exp(k v F/RT) - 1) is 0/0 whenever a cell's Vm
+// rounds to within ~1e-6 mV of 0 in float (millions of plateau cells cross 0 mV),
+// while in FP64 that window is ~1e-15 mV.
+template <typename T> __device__ __forceinline__ T x_over_expm1(T x) { return x == T(0) ? T(1) : x / dexpm1<T>(x); }
 
 // Rush-Larsen update toward the steady state: inf + (x - inf) * exp(-dt / tau)
 template <typename T> __device__ __forceinline__ T rl_toward(T inf, T x, T dt, T tau) {
@@ -384,9 +411,18 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt) {
     nca = anca * k2n / km2n - (anca * k2n / km2n - nca) * dexp<T>(-km2n * dt);
     const T e2 = dexp<T>(T(2.0f) * vfrt);
     const T e1 = dexp<T>(T(1.0f) * vfrt);
-    const T PhiCaL = T(4.0f) * vffrt * (cass * e2 - T(0.341f) * cao) / (e2 - T(1.0f));
-    const T PhiCaNa = T(1.0f) * vffrt * (T(0.75f) * nass * e1 - T(0.75f) * nao) / (e1 - T(1.0f));
-    const T PhiCaK = T(1.0f) * vffrt * (T(0.75f) * kss * e1 - T(0.75f) * ko) / (e1 - T(1.0f));
+    T PhiCaL, PhiCaNa, PhiCaK;
+    if constexpr(sizeof(T) == 8) {
+        PhiCaL = T(4.0f) * vffrt * (cass * e2 - T(0.341f) * cao) / (e2 - T(1.0f));
+        PhiCaNa = T(1.0f) * vffrt * (T(0.75f) * nass * e1 - T(0.75f) * nao) / (e1 - T(1.0f));
+        PhiCaK = T(1.0f) * vffrt * (T(0.75f) * kss * e1 - T(0.75f) * ko) / (e1 - T(1.0f));
+    } else {
+        // 4 vffrt / (e2 - 1) = 2 F * (2 vfrt) / expm1(2 vfrt), likewise for e1
+        const T g2 = T(2.0f) * F * x_over_expm1<T>(T(2.0f) * vfrt), g1 = F * x_over_expm1<T>(vfrt);
+        PhiCaL = g2 * (cass * e2 - T(0.341f) * cao);
+        PhiCaNa = g1 * (T(0.75f) * nass * e1 - T(0.75f) * nao);
+        PhiCaK = g1 * (T(0.75f) * kss * e1 - T(0.75f) * ko);
+    }
     const T zca = T(2.0f);
     const T PCa = T(0.0001);
     const T PCap = T(1.1f) * PCa;
@@ -524,9 +560,15 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt) {
     const T IKb = GKb * xkb * (v - EK);
     const T PNab = T(3.75e-10);
     const T ev = dexp<T>(vfrt);
-    const T INab = PNab * vffrt * (nai * ev - nao) / (ev - T(1.0f));
     const T PCab = T(2.5e-8);
-    const T ICab = PCab * T(4.0f) * vffrt * (cai * e2 - T(0.341f) * cao) / (e2 - T(1.0f));
+    T INab, ICab;
+    if constexpr(sizeof(T) == 8) {
+        INab = PNab * vffrt * (nai * ev - nao) / (ev - T(1.0f));
+        ICab = PCab * T(4.0f) * vffrt * (cai * e2 - T(0.341f) * cao) / (e2 - T(1.0f));
+    } else {
+        INab = PNab * F * x_over_expm1<T>(vfrt) * (nai * ev - nao);
+        ICab = PCab * T(2.0f) * F * x_over_expm1<T>(T(2.0f) * vfrt) * (cai * e2 - T(0.341f) * cao);
+    }
     const T GpCa = T(0.0005);
     const T IpCa = GpCa * cai / (T(0.0005f) + cai);
 

@@ -10,7 +10,10 @@
 //     Vm falls to the APD threshold (or at once for the earlier of two pending
 //     activations).
 // The reference keeps Vm, counters and times in float (its hash values and
-// arrays are float); so does this kernel.  Times and APDs beyond max_events per
+// arrays are float); so does this kernel, except that the previous Vm can be
+// kept in FP64 instead (vm_double): with a float copy, a sample that rounds to
+// exactly the threshold from below hides that cell's activation (about one
+// cell in a million), which differs between two runs that agree to 1e-6 mV.  Times and APDs beyond max_events per
 // cell are counted but not stored (their APDs come out as NaN).
 #include "mono_common.h"
 #include "observer.h"
@@ -21,20 +24,21 @@ namespace mono {
 
 __global__ __launch_bounds__(256) void k_observe(int64_t n, const double *__restrict__ v, const uint8_t *__restrict__ active, double t,
                                                  float act_thr, float apd_thr, float time_thr, int kmax, float *__restrict__ last_v,
-                                                 float *__restrict__ was_active, int32_t *__restrict__ n_act, int32_t *__restrict__ n_apd,
-                                                 float *__restrict__ act, float *__restrict__ apd) {
+                                                 double *__restrict__ last_vd, float *__restrict__ was_active, int32_t *__restrict__ n_act,
+                                                 int32_t *__restrict__ n_apd, float *__restrict__ act, float *__restrict__ apd) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= n) return;
     if(active && !active[i]) return;
     const double vi = v[i];
     if(t == 0.0) {
-        last_v[i] = (float)vi;
+        if(last_vd) last_vd[i] = vi;
+        else last_v[i] = (float)vi;
         return;
     }
-    const float lv = last_v[i];
     int na = n_act[i];
     float wa = was_active[i];
-    if(lv < act_thr && vi >= (double)act_thr) {
+    const bool below = last_vd ? last_vd[i] < (double)act_thr : last_v[i] < act_thr;
+    if(below && vi >= (double)act_thr) {
         bool rec = na == 0;
         if(!rec) {
             const double last = na - 1 < kmax ? (double)act[(int64_t)(na - 1) * n + i] : (double)NAN;
@@ -55,14 +59,16 @@ __global__ __launch_bounds__(256) void k_observe(int64_t n, const double *__rest
         n_apd[i] = np + 1;
         wa = (float)(tmp - 1);
     }
-    last_v[i] = (float)vi;
+    if(last_vd) last_vd[i] = vi;
+    else last_v[i] = (float)vi;
     n_act[i] = na;
     was_active[i] = wa;
 }
 
-int Observer::init(int64_t n_, float a, float p, float tt, int kmax, hipStream_t s) {
+int Observer::init(int64_t n_, float a, float p, float tt, int kmax, int vmd, hipStream_t s) {
     if(release()) return -1;
     n = n_;
+    vm_double = vmd;
     act_thr = a;
     apd_thr = p;
     time_thr = tt;
@@ -78,6 +84,12 @@ int Observer::init(int64_t n_, float a, float p, float tt, int kmax, hipStream_t
     const float m100 = -100.0f;
     std::vector<float> h(nn, m100);
     MONO_CHECK(hipMemcpyAsync(last_v, h.data(), sizeof(float) * nn, hipMemcpyHostToDevice, s));
+    if(vmd) {
+        MONO_CHECK(hipMalloc(&last_vd, sizeof(double) * nn));
+        std::vector<double> hd(nn, -100.0);
+        MONO_CHECK(hipMemcpyAsync(last_vd, hd.data(), sizeof(double) * nn, hipMemcpyHostToDevice, s));
+        MONO_CHECK(hipStreamSynchronize(s));
+    }
     MONO_CHECK(hipMemsetAsync(was_active, 0, sizeof(float) * nn, s));
     MONO_CHECK(hipMemsetAsync(n_act, 0, sizeof(int32_t) * nn, s));
     MONO_CHECK(hipMemsetAsync(n_apd, 0, sizeof(int32_t) * nn, s));
@@ -88,10 +100,11 @@ int Observer::init(int64_t n_, float a, float p, float tt, int kmax, hipStream_t
 }
 
 int Observer::release() {
-    void *bufs[] = {last_v, was_active, n_act, n_apd, act, apd};
+    void *bufs[] = {last_v, was_active, n_act, n_apd, act, apd, last_vd};
     for(void *p : bufs)
         if(p) MONO_CHECK(hipFree(p));
     last_v = was_active = act = apd = nullptr;
+    last_vd = nullptr;
     n_act = n_apd = nullptr;
     max_events = 0;
     return 0;
@@ -103,8 +116,8 @@ int Observer::observe(const double *v, const uint8_t *active, double t, hipStrea
         return -1;
     }
     if(n == 0) return 0;
-    k_observe<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, v, active, t, act_thr, apd_thr, time_thr, max_events, last_v, was_active,
-                                                          n_act, n_apd, act, apd);
+    k_observe<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, v, active, t, act_thr, apd_thr, time_thr, max_events, last_v, last_vd,
+                                                          was_active, n_act, n_apd, act, apd);
     MONO_CHECK(hipGetLastError());
     return 0;
 }

@@ -282,12 +282,12 @@ int mono_tissue_destroy(mono_tissue *t) {
 }
 
 int mono_tissue_observe_setup(mono_tissue *t, float activation_threshold, float apd_threshold, float time_threshold,
-                              int max_events) {
+                              int max_events, int vm_double) {
     if(!t || max_events < 1 || max_events > 1024) {
         mono_set_error("mono_tissue_observe_setup: bad arguments");
         return -1;
     }
-    return t->obs.init(t->n, activation_threshold, apd_threshold, time_threshold, max_events, t->stream);
+    return t->obs.init(t->n, activation_threshold, apd_threshold, time_threshold, max_events, vm_double, t->stream);
 }
 
 int mono_tissue_observe(mono_tissue *t) {

@@ -89,3 +89,34 @@ def test_fp32_action_potential_shape(oracle, name, amp):
     vr, vg = np.array(vr), np.array(vg)
     span = vr.max() - vr.min()
     assert np.abs(vg - vr).max() < 0.05 * span, np.abs(vg - vr).max()
+
+
+@pytest.mark.parametrize("name", ["tt2006", "ord2011"])
+def test_fp32_finite_through_zero_mV(oracle, name):
+    """Cells whose potential sits at (or within rounding of) 0 mV: the FP32
+    kernel must stay finite where the reference's GHK factors
+    vF^2/RT * (...) / (exp(k vF/RT) - 1) are 0/0 in float; FP64 and FP32
+    one-step results agree to FP32 accuracy there."""
+    import monoalg3d_c_amd as m
+    announce(f"ionic fp32 near 0 mV {name}")
+    mid = m.MODELS[name]
+    sv = oracle.model_init(mid, 1)
+    stim = _stim_schedule(4000, 1, -50.0)
+    plateau = None
+    for k in range(4000):
+        oracle.model_solve(mid, sv, stim[k], DT, 1)
+        if k > 150 and plateau is None and sv[0, 0] > 5.0:
+            plateau = sv[0].copy()
+    assert plateau is not None
+    vs = np.array([0.0, 1e-9, -1e-9, 1e-7, -1e-7, 3e-6, -3e-6, 1e-3])
+    X = np.tile(plateau, (len(vs), 1))
+    X[:, 0] = vs
+    g32 = m.ionic_run(mid, X, None, DT, 1, precision=m.FP32_REFINE)
+    g64 = m.ionic_run(mid, X, None, DT, 1)
+    assert np.all(np.isfinite(g32)), g32[:, 0]
+    # FP64 keeps the reference's form, which is 0/0 at exactly 0 mV (as in the
+    # reference); compare where it is finite
+    ok = np.all(np.isfinite(g64), axis=1)
+    assert ok.sum() >= len(vs) - 1
+    rel = np.abs(g32[ok] - g64[ok]) / np.maximum(np.abs(g64[ok]), 1e-6)
+    assert rel.max() < 1e-3, rel.max()
