@@ -248,17 +248,17 @@ __device__ __forceinline__ TV row_lds(const TC *row, const TV *const (&P3)[3], i
     return acc;
 }
 
-// 27-slot class row (slot = (oz+1)*9 + (oy+1)*3 + (ox+1)) against the LDS
-// planes, one oz plane at a time so these rare rows do not raise the kernel's
-// register count.  R is a scalar (wave-uniform) or per-lane row pointer.
-template <typename R, typename TV>
-__device__ __forceinline__ TV row27_lds(R row, const TV *const (&P3)[3], int base) {
+// 27-slot class row (slot = (oz+1)*9 + (oy+1)*3 + (ox+1)), staged in LDS,
+// against the LDS planes: one oz plane (9 slots) at a time, so these rows do
+// not raise the kernel's register count.
+template <typename TC, typename TV>
+__device__ __forceinline__ TV row27_lds(const TC *row, const TV *const (&P3)[3], int base) {
     TV acc = TV(0);
 #pragma unroll 1
     for(int oz = 0; oz < 3; oz++) {
         const TV *P = (oz == 0 ? P3[0] : (oz == 1 ? P3[1] : P3[2])) + base;
-        const R r = row + 9 * oz;
-#pragma unroll 3
+        const TC *r = row + 9 * oz;
+#pragma unroll
         for(int j = 0; j < 9; j++) acc += (TV)r[j] * P[(j / 3 - 1) * kW + (j % 3 - 1)];
     }
     return acc;
@@ -269,7 +269,9 @@ template <typename TV, typename TC, bool CLS, int PAT, bool FUSE, class Epi>
 __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeom &g, int t, int z0, int z1, const TV *__restrict__ x,
                                              const TV *__restrict__ rz, TV beta, TV *__restrict__ p_out, const Epi &epi,
                                              double (&v)[Epi::NR]) {
+    static_assert(9 * 27 <= kBlock, "one class-row coefficient per thread");
     __shared__ TV pl[3 * kPL];
+    __shared__ TC crows[CLS ? 9 * 27 : 1];
     const int x0 = (t % g.ntx) * kTX, y0 = (t / g.ntx) * kTY;
     const int tid = threadIdx.x;
     const int64_t sz = (int64_t)op.nx * op.ny;
@@ -300,20 +302,38 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
         for(int j = 0; j < kNLD; j++)
             if(tid + j * kBlock < kPL) pl[slot * kPL + tid + j * kBlock] = reg[j];
     };
+    // class mode: the 9 class rows of a plane (27 slots each) are staged in LDS,
+    // one plane ahead, for tiles on an x or y face and on the boundary planes
+    const bool face_xy = CLS && (x0 < g.lo[0] || x0 + kTX - 1 > g.hi[0] || y0 < g.lo[1] || y0 + kTY - 1 > g.hi[1]);
+    auto need_rows = [&](int z) { return CLS && (face_xy || z < g.lo[2] || z > g.hi[2]); };
+    TC rrow = TC(0);
+    auto fetch_rows = [&](int z) {
+        if(tid < 9 * 27 && need_rows(z)) rrow = op.table[(int64_t)z * 9 * 27 + tid];
+    };
     const int tx = tid % kTX, ty = tid / kTX;
     const int gx = x0 + tx;
+    const bool gx_in = gx < op.nx;
     // class mode: lanes on an x face are evaluated by the face pass below
     const bool xf = !CLS || (gx >= g.lo[0] && gx <= g.hi[0]);
     // face pass: lanes 0..15 of wave 0 take the tile's x-face cells (row l & 7,
     // left face for l < 8 when the tile holds x = 0, right face otherwise)
     const int fx = tid < 8 ? 0 : op.nx - 1;
     const bool face_lane = CLS && tid < 16 && fx >= x0 && fx < x0 + kTX && (fx < g.lo[0] || fx > g.hi[0]) && y0 + (tid & 7) < op.ny;
-    auto finish = [&](int64_t c, TV acc, TV xc, double dg) {
-        if constexpr(FUSE) p_out[c] = xc;
-        epi(c, acc, xc, dg, v);
+    auto ycls = [&](int gy) { return gy == 0 ? 0 : (gy == op.ny - 1 ? 2 : 1); };
+    // epilogue inputs of plane z (e.g. b), loaded a plane ahead
+    using Pre = typename Epi::Pre;
+    Pre pa[kRPT], pb[kRPT];
+    auto cell_of = [&](int q, int z) -> int64_t {
+        const int gy = y0 + ty + q * (kBlock / kTX);
+        return (gx_in && gy < op.ny) ? gx + (int64_t)op.nx * gy + sz * z : 0;
     };
-    // plane z from LDS (planes z-1, z, z+1 staged)
-    auto compute = [&](int z) {
+    auto preload = [&](Pre(&pre)[kRPT], int z) {
+#pragma unroll
+        for(int q = 0; q < kRPT; q++) pre[q] = epi.load(cell_of(q, z));
+    };
+    auto face_pre = [&](int z) -> Pre { return epi.load(face_lane ? fx + (int64_t)op.nx * (y0 + (tid & 7)) + sz * z : 0); };
+    // plane z from LDS (planes z-1, z, z+1 and, when needed, its class rows staged)
+    auto compute = [&](int z, const Pre(&pre)[kRPT], Pre fpre) {
         const TV *const P3[3] = {pl + ((z + 2) % 3) * kPL, pl + (z % 3) * kPL, pl + ((z + 1) % 3) * kPL};
         TC crow[PAT];
         bool zf = true;
@@ -330,7 +350,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
         for(int q = 0; q < kRPT; q++) {
             const int rr = ty + q * (kBlock / kTX);
             const int gy = y0 + rr;
-            if(gx >= op.nx || gy >= op.ny) continue;
+            if(!gx_in || gy >= op.ny) continue;
             const int64_t c = gx + (int64_t)op.nx * gy + sz * z;
             const int base = (rr + 1) * kW + tx + 1;
             TV acc;
@@ -346,28 +366,31 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
                            P3[RowPattern<PAT>::o[k][2] + 1][base + RowPattern<PAT>::o[k][1] * kW + RowPattern<PAT>::o[k][0]];
                 if constexpr(Epi::kDiag) dg = (double)op.percell[cu];
             } else {
+                if(!xf) continue;
                 if(zf && gy >= g.lo[1] && gy <= g.hi[1]) {
                     acc = row_lds<PAT>(crow, P3, base);
                     if constexpr(Epi::kDiag) dg = (double)crow[0];
                 } else {
-                    // y face or boundary plane: the row of class (z, y class, interior x)
-                    const int cyc = gy == 0 ? 0 : (gy == op.ny - 1 ? 2 : 1), cxc = op.nx == 1 ? 0 : 1;
-                    const scalar_ptr<TC> row = as_scalar(op.table + ((z * 3 + cyc) * 3 + cxc) * 27);
+                    // y face or boundary plane: the row of class (y class, interior x),
+                    // the same for the whole wave
+                    const TC *row = crows + (ycls(gy) * 3 + (op.nx == 1 ? 0 : 1)) * 27;
                     acc = row27_lds(row, P3, base);
                     if constexpr(Epi::kDiag) dg = (double)row[13];
                 }
-                if(!xf) continue;
             }
-            finish(c, acc, P3[1][base], dg);
+            const TV xc = P3[1][base];
+            if constexpr(FUSE) p_out[c] = xc;
+            epi(c, acc, xc, dg, pre[q], v);
         }
         if constexpr(CLS) {
             if(face_lane) {
                 const int rr = tid & 7, gy = y0 + rr;
                 const int64_t c = fx + (int64_t)op.nx * gy + sz * z;
                 const int base = (rr + 1) * kW + (fx - x0) + 1;
-                const int cyc = gy == 0 ? 0 : (gy == op.ny - 1 ? 2 : 1), cxc = fx == 0 ? 0 : 2;
-                const TC *row = op.table + ((z * 3 + cyc) * 3 + cxc) * 27;
-                finish(c, row27_lds(row, P3, base), P3[1][base], (double)row[13]);
+                const TC *row = crows + (ycls(gy) * 3 + (fx == 0 ? 0 : 2)) * 27;
+                const TV xc = P3[1][base];
+                if constexpr(FUSE) p_out[c] = xc;
+                epi(c, row27_lds(row, P3, base), xc, (double)row[13], fpre, v);
             }
         }
     };
@@ -377,17 +400,36 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
     put(ra, z0 % 3);
     fetch(ra, z0 + 1);
     if(z0 + 2 <= z1) fetch(rb, z0 + 2);
+    fetch_rows(z0);
+    preload(pa, z0);
+    Pre fa = face_pre(z0), fb{};
+    if(z0 + 1 < z1) {
+        preload(pb, z0 + 1);
+        fb = face_pre(z0 + 1);
+    }
     for(int z = z0; z < z1; z += 2) {
         put(ra, (z + 1) % 3);  // plane z+1
+        if(need_rows(z) && tid < 9 * 27) crows[tid] = rrow;
         if(z + 3 <= z1) fetch(ra, z + 3);
+        if(z + 1 < z1) fetch_rows(z + 1);
         __syncthreads();
-        compute(z);
+        compute(z, pa, fa);
+        if(z + 2 < z1) {
+            preload(pa, z + 2);
+            fa = face_pre(z + 2);
+        }
         __syncthreads();
         if(z + 1 >= z1) break;
         put(rb, (z + 2) % 3);  // plane z+2
+        if(need_rows(z + 1) && tid < 9 * 27) crows[tid] = rrow;
         if(z + 4 <= z1) fetch(rb, z + 4);
+        if(z + 2 < z1) fetch_rows(z + 2);
         __syncthreads();
-        compute(z + 1);
+        compute(z + 1, pb, fb);
+        if(z + 3 < z1) {
+            preload(pb, z + 3);
+            fb = face_pre(z + 3);
+        }
         __syncthreads();
     }
 }
@@ -437,11 +479,17 @@ __global__ __launch_bounds__(kBlock) void k_stencil(OpView<TC> op, TileGeom g, c
 }
 
 // q = A p, partial p.q
+// Epilogue interface: NR partial sums; kDiag: needs the row's diagonal;
+// Pre load(c): the epilogue's inputs of cell c, loaded a plane ahead.
+struct NoPre {};
+
 template <typename TV> struct EpiSpmv {
     static constexpr int NR = 1;
     static constexpr bool kDiag = false;
+    using Pre = NoPre;
     TV *q;
-    __device__ void operator()(int64_t c, TV ax, TV pc, double, double (&v)[NR]) const {
+    __device__ Pre load(int64_t) const { return {}; }
+    __device__ void operator()(int64_t c, TV ax, TV pc, double, Pre, double (&v)[NR]) const {
         q[c] = ax;
         v[0] += (double)pc * (double)ax;
     }
@@ -451,10 +499,11 @@ template <typename TV> struct EpiSpmv {
 template <bool JACOBI> struct EpiRes64 {
     static constexpr int NR = 3;
     static constexpr bool kDiag = JACOBI;
+    using Pre = double;
     const double *b;
     double *r, *p, *z;
-    __device__ void operator()(int64_t c, double ax, double, double dg, double (&v)[NR]) const {
-        const double bc = b[c];
+    __device__ Pre load(int64_t c) const { return b[c]; }
+    __device__ void operator()(int64_t c, double ax, double, double dg, Pre bc, double (&v)[NR]) const {
         const double rc = bc - ax;
         r[c] = rc;
         if constexpr(JACOBI) {
@@ -475,10 +524,11 @@ template <bool JACOBI> struct EpiRes64 {
 struct EpiResOuter {
     static constexpr int NR = 2;  // r.r, b.b
     static constexpr bool kDiag = false;
+    using Pre = double;
     const double *b;
     float *r32, *p32, *d32;
-    __device__ void operator()(int64_t c, double ax, double, double, double (&v)[NR]) const {
-        const double bc = b[c];
+    __device__ Pre load(int64_t c) const { return b[c]; }
+    __device__ void operator()(int64_t c, double ax, double, double, Pre bc, double (&v)[NR]) const {
         const double rc = bc - ax;
         const float rf = (float)rc;
         r32[c] = rf;
