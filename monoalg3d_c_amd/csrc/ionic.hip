@@ -36,7 +36,7 @@ __device__ __forceinline__ T model_step(T *y, T stim, T dt, const float *scale) 
 }
 
 template <int MODEL, typename T>
-__global__ __launch_bounds__(256) void k_ionic_step(IonicLaunch<T> a) {
+__device__ __forceinline__ void ionic_step_body(const IonicLaunch<T> &a) {
     constexpr int NEQ = ModelTraits<MODEL>::neq;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= a.n) return;
@@ -71,6 +71,19 @@ __global__ __launch_bounds__(256) void k_ionic_step(IonicLaunch<T> a) {
     if(a.write_vm_row) a.sv[i] = T(V);
     // fused UPDATE_MONODOMAIN: b = Vm * alpha
     if(a.b) a.b[i] = V * a.alpha;
+}
+
+// FP32: the register count the compiler picks.  FP64 models are bound by
+// their transcendental and division arithmetic; at two waves per SIMD the
+// state loads of one wave overlap the arithmetic of the other (a few
+// registers spill to scratch, which measures faster than one wave per SIMD).
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_ionic_step32(IonicLaunch<float> a) {
+    ionic_step_body<MODEL, float>(a);
+}
+template <int MODEL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_ionic_step64(IonicLaunch<double> a) {
+    ionic_step_body<MODEL, double>(a);
 }
 
 template <int MODEL, typename T>
@@ -115,11 +128,20 @@ static hipError_t launch_step_t(int model, const IonicLaunch<T> &a, hipStream_t 
     const int block = 256;
     const unsigned grid = (unsigned)((a.n + block - 1) / block);
     if(a.n == 0) return hipSuccess;
-    switch(model) {
-    case kTT2006: k_ionic_step<kTT2006, T><<<grid, block, 0, s>>>(a); break;
-    case kORd2011: k_ionic_step<kORd2011, T><<<grid, block, 0, s>>>(a); break;
-    case kMS2003: k_ionic_step<kMS2003, T><<<grid, block, 0, s>>>(a); break;
-    default: return hipErrorInvalidValue;
+    if constexpr(sizeof(T) == 8) {
+        switch(model) {
+        case kTT2006: k_ionic_step64<kTT2006><<<grid, block, 0, s>>>(a); break;
+        case kORd2011: k_ionic_step64<kORd2011><<<grid, block, 0, s>>>(a); break;
+        case kMS2003: k_ionic_step64<kMS2003><<<grid, block, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch(model) {
+        case kTT2006: k_ionic_step32<kTT2006><<<grid, block, 0, s>>>(a); break;
+        case kORd2011: k_ionic_step32<kORd2011><<<grid, block, 0, s>>>(a); break;
+        case kMS2003: k_ionic_step32<kMS2003><<<grid, block, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+        }
     }
     return hipGetLastError();
 }
