@@ -126,6 +126,25 @@ def lat_error(r64, r32, lat_ms):
                 max_abs_err_ms=float(err) if (both.any() or one.any()) else None)
 
 
+def copy_calibration(mb):
+    """Three device-to-device copies of `mb` MiB through the HIP runtime (with
+    HSA_ENABLE_SDMA=0 in the environment they run as the runtime's copy kernel):
+    the known-size reference for the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = C.c_size_t(mb << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), n) or hip.hipMalloc(C.byref(b), n):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 0, n)
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, n, 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def measured_traffic(kernel, shape):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, each
@@ -260,28 +279,21 @@ def main() -> int:
                                       and lat_check["nan_step_fp32"] is None and lat_check["nan_step_fp64"] is None)
         print(f"[bench] lat check {lat_check}", file=sys.stderr, flush=True)
     if args.copy_calib_mb > 0:
-        import torch
-        nbytes = args.copy_calib_mb << 20
-        # an element-wise kernel (a plain copy_ would go to a DMA engine): reads
-        # nbytes, writes nbytes
-        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            torch.mul(a, 1.0, out=b)
-        torch.cuda.synchronize()
+        copy_calibration(args.copy_calib_mb)
 
-    # roofline of the dominant phase (by device time)
+    # per-phase rooflines (algorithmic bytes / device time of the phase)
     rf = {}
     for key, r in res.items():
         w = 8 if key == "fp64" else 4
+        k_it = r["cg_its"] if key == "fp64" else r["inner_its"]
+        step_b = bytes_per_cell_step(neq, w, k_it, key == "fp32", r["outer"])
         ionic_bytes = (2 * (neq - 1) * w + 16) * n
-        solve_bytes = (bytes_per_cell_step(neq, w, r["cg_its"] if key == "fp64" else r["inner_its"], key == "fp32",
-                                           r["outer"]) - (2 * (neq - 1) * w + 16)) * n
+        solve_bytes = (step_b - (2 * (neq - 1) * w + 16)) * n
         rf[key] = dict(
             ionic=dict(achieved=ionic_bytes / (r["ionic_ms"] * 1e-3) / 1e9, bytes=ionic_bytes, ms=r["ionic_ms"]),
             solve=dict(achieved=solve_bytes / (r["solve_ms"] * 1e-3) / 1e9, bytes=solve_bytes, ms=r["solve_ms"]),
-            step_bytes_per_cell=bytes_per_cell_step(neq, w, r["cg_its"] if key == "fp64" else r["inner_its"],
-                                                    key == "fp32", r["outer"]))
+            step_bytes_per_cell=step_b)
+
     # The dominant kernel is the ionic step: one launch per step, more device
     # time than any one kernel of the solve (whose phase time spreads over the
     # stencil, update and residual kernels; see profiles/).  Its launch time is
