@@ -96,6 +96,13 @@ typedef struct mono_tissue_desc {
     double inner_rel;               /* FP32_REFINE: inner relative tolerance (0 = 1e-4) */
     /* stimulus ([stim_*] stim_if_x_less_than) */
     double stim_start, stim_duration, stim_current, stim_x_limit, stim_period;
+    /* optional host per-cell 0/1: the cells the stimulus reaches (replaces the
+     * x < stim_x_limit rule; e.g. one region per instance of a batch) */
+    const uint8_t *stim_mask;
+    /* optional host per-cell model scalings, 4 floats per cell: TT2006 GKr,
+     * GCaL, GKs (4th unused); MS2003 tau_close (2nd-4th unused); all 1 is the
+     * reference model.  No counterpart in the reference (batch sweeps). */
+    const float *ionic_scale;
 } mono_tissue_desc;
 
 typedef struct mono_stats {

@@ -76,8 +76,11 @@ def layered_fibre_tensors(nz: int, theta0_deg: float, theta1_deg: float, sigma_l
 def make_desc(shape, h=(100.0, 100.0, 100.0), model=MODEL_TT2006, precision=FP64, dt=0.02, dt_ode=None, beta=0.14,
               cm=1.0, aniso=False, tensor=None, tensor_source=TENSOR_UNIFORM, g_cell=None, active=None,
               cg_tol=1e-16, cg_rel_tol=0.0, cg_max_its=200, jacobi=False, inner_rel=1e-4,
-              stim=(0.0, 2.0, -50.0, 500.0, 0.0)):
-    """Build a TissueDesc; returns (desc, keepalive) where keepalive holds the arrays."""
+              stim=(0.0, 2.0, -50.0, 500.0, 0.0), stim_mask=None, ionic_scale=None):
+    """Build a TissueDesc; returns (desc, keepalive) where keepalive holds the arrays.
+
+    stim_mask (n,) 0/1 replaces the x < stim_x_limit rule; ionic_scale (n, 4)
+    float32 scales TT2006 GKr/GCaL/GKs or MS2003 tau_close per cell."""
     d = TissueDesc()
     d.nx, d.ny, d.nz = shape
     d.dx, d.dy, d.dz = h
@@ -99,6 +102,19 @@ def make_desc(shape, h=(100.0, 100.0, 100.0), model=MODEL_TT2006, precision=FP64
         active = np.ascontiguousarray(active, np.uint8)
         keep.append(active)
         d.active = active.ctypes.data
+    n = int(d.nx) * int(d.ny) * int(d.nz)
+    if stim_mask is not None:
+        stim_mask = np.ascontiguousarray(stim_mask, np.uint8).reshape(-1)
+        if stim_mask.size != n:
+            raise ValueError(f"stim_mask has {stim_mask.size} entries for {n} cells")
+        keep.append(stim_mask)
+        d.stim_mask = stim_mask.ctypes.data
+    if ionic_scale is not None:
+        ionic_scale = np.ascontiguousarray(ionic_scale, np.float32).reshape(-1)
+        if ionic_scale.size != 4 * n:
+            raise ValueError(f"ionic_scale has {ionic_scale.size} entries for 4 x {n} cells")
+        keep.append(ionic_scale)
+        d.ionic_scale = ionic_scale.ctypes.data
     d.cg_tol, d.cg_rel_tol, d.cg_max_its, d.cg_jacobi = cg_tol, cg_rel_tol, cg_max_its, int(jacobi)
     d.inner_rel = inner_rel
     d.stim_start, d.stim_duration, d.stim_current, d.stim_x_limit, d.stim_period = stim

@@ -45,6 +45,7 @@ class TissueDesc(C.Structure):
         ("inner_rel", C.c_double),
         ("stim_start", C.c_double), ("stim_duration", C.c_double), ("stim_current", C.c_double),
         ("stim_x_limit", C.c_double), ("stim_period", C.c_double),
+        ("stim_mask", C.c_void_p), ("ionic_scale", C.c_void_p),
     ]
 
 

@@ -14,6 +14,7 @@ template <typename T> struct IonicLaunch {
     int64_t n;              // cells
     const uint8_t *active;  // optional 0/1 per cell
     const double *stim_cell;  // optional per-cell stimulus (overrides the x-limit rule)
+    const uint8_t *stim_mask; // optional per-cell 0/1: stimulated cells (replaces the x-limit rule)
     const float *scale;     // optional per-cell conductance scalings, 4 per cell
     int nx;                 // cells along x (for the centre of a cell)
     double dx;              // cell size along x (um)

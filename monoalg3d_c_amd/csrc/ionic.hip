@@ -47,14 +47,19 @@ __device__ __forceinline__ void ionic_step_body(const IonicLaunch<T> &a) {
     for(int s = 1; s < NEQ; s++) y[s] = a.sv[(size_t)s * a.pitch + i];
     double V = a.first_step ? (double)a.sv[i] : a.v[i];
 
-    // stimulus for this cell: per-cell array, else x-limit mask on the centre
+    // stimulus for this cell: per-cell array, else (inside the stimulus window)
+    // the per-cell mask or the x-limit rule on the cell centre
     double st = 0.0;
     if(a.stim_cell) {
         st = a.stim_cell[i];
     } else if(a.stim_amp != 0.0) {
-        const int ix = (int)(i % a.nx);
-        const double cx = ix * a.dx + a.dx / 2.0;
-        if(cx < a.stim_x_limit) st = a.stim_amp;
+        if(a.stim_mask) {
+            if(a.stim_mask[i]) st = a.stim_amp;
+        } else {
+            const int ix = (int)(i % a.nx);
+            const double cx = ix * a.dx + a.dx / 2.0;
+            if(cx < a.stim_x_limit) st = a.stim_amp;
+        }
     }
     float scale[4] = {1.f, 1.f, 1.f, 1.f};
     if(a.scale) {

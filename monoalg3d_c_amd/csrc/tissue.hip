@@ -42,6 +42,8 @@ struct mono_tissue {
     uint8_t *active = nullptr;
     double *g = nullptr;
     double *tensor = nullptr;
+    uint8_t *stim_mask = nullptr;
+    float *ionic_scale = nullptr;
     Operator op;
     CgWorkspace cg;
     CgConfig cgcfg;
@@ -167,6 +169,8 @@ int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out) {
     t->d.tensor = nullptr;
     t->d.g_cell = nullptr;
     t->d.active = nullptr;
+    t->d.stim_mask = nullptr;
+    t->d.ionic_scale = nullptr;
     const mono_tissue_desc &d = *desc;
     t->n = (int64_t)d.nx * d.ny * d.nz;
     t->neq = model_neq(d.model);
@@ -203,6 +207,22 @@ int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out) {
         mono_tissue_destroy(t);
         return -1;
     };
+
+    // per-cell stimulus mask and model scalings
+    if(d.stim_mask) {
+        if(hipMalloc(&t->stim_mask, t->n) != hipSuccess ||
+           hipMemcpyAsync(t->stim_mask, d.stim_mask, t->n, hipMemcpyHostToDevice, s) != hipSuccess) {
+            mono_set_error("stimulus mask upload failed");
+            return fail();
+        }
+    }
+    if(d.ionic_scale) {
+        if(hipMalloc(&t->ionic_scale, sizeof(float) * 4 * t->n) != hipSuccess ||
+           hipMemcpyAsync(t->ionic_scale, d.ionic_scale, sizeof(float) * 4 * t->n, hipMemcpyHostToDevice, s) != hipSuccess) {
+            mono_set_error("ionic scale upload failed");
+            return fail();
+        }
+    }
 
     // operator
     double *coef27 = nullptr;
@@ -271,7 +291,7 @@ int mono_tissue_destroy(mono_tissue *t) {
     if(t->cg.release()) rc = -1;
     if(t->op.release()) rc = -1;
     if(t->obs.release()) rc = -1;
-    void *bufs[] = {t->sv, t->v, t->b, t->active, t->g, t->tensor};
+    void *bufs[] = {t->sv, t->v, t->b, t->active, t->g, t->tensor, t->stim_mask, t->ionic_scale};
     for(void *p : bufs)
         if(p && hipFree(p) != hipSuccess) rc = -1;
     for(int k = 0; k < 3; k++)
@@ -331,6 +351,8 @@ static int launch_ionic(mono_tissue *t, double stim_amp) {
         a.b = t->b;
         a.n = t->n;
         a.active = t->active;
+        a.stim_mask = t->stim_mask;
+        a.scale = t->ionic_scale;
         a.nx = d.nx;
         a.dx = d.dx;
         a.stim_x_limit = d.stim_x_limit;
@@ -348,6 +370,8 @@ static int launch_ionic(mono_tissue *t, double stim_amp) {
         a.b = t->b;
         a.n = t->n;
         a.active = t->active;
+        a.stim_mask = t->stim_mask;
+        a.scale = t->ionic_scale;
         a.nx = d.nx;
         a.dx = d.dx;
         a.stim_x_limit = d.stim_x_limit;

@@ -15,7 +15,7 @@
 #include "oracle.h"
 
 void orc_tt2006_init_cell(double *y);
-void orc_tt2006_step(double *y, double stim, double dt);
+void orc_tt2006_step_scaled(double *y, double stim, double dt, double sKr, double sCaL, double sKs);
 void orc_ord2011_init_cell(double *y);
 void orc_ord2011_step(double *y, double stim, double dt);
 
@@ -24,9 +24,11 @@ static void ms_init_cell(double *y) {
     y[1] = (double)0.8789655121804799f;
 }
 
-static void ms_step(double *y, double stim, double dt) {
+/* s_tau_close multiplies tau_close (no reference counterpart; 1 is the
+ * reference model) */
+static void ms_step(double *y, double stim, double dt, double s_tau_close) {
     const double V = y[0], h = y[1];
-    const double tau_in = 0.3, tau_out = 6.0, V_gate = 0.13, tau_open = 120.0, tau_close = 150.0;
+    const double tau_in = 0.3, tau_out = 6.0, V_gate = 0.13, tau_open = 120.0, tau_close = 150.0 * s_tau_close;
     const double J_in = (h * (pow(V, 2.00000) * (1.00000 - V))) / tau_in;
     const double J_out = -(V / tau_out);
     const double dV = J_out + J_in + stim;
@@ -65,16 +67,23 @@ void orc_model_init(int model, double *sv, int64_t n) {
     }
 }
 
-void orc_model_solve(int model, double *sv, int64_t n, const double *stim, double dt, int num_steps) {
+void orc_model_solve_scaled(int model, double *sv, int64_t n, const double *stim, const float *scale, double dt,
+                            int num_steps) {
     const int neq = orc_model_neq(model);
 #pragma omp parallel for schedule(static)
     for(int64_t i = 0; i < n; i++) {
         double *y = sv + i * neq;
         const double st = stim ? stim[i] : 0.0;
+        const double s0 = scale ? (double)scale[4 * i] : 1.0, s1 = scale ? (double)scale[4 * i + 1] : 1.0;
+        const double s2 = scale ? (double)scale[4 * i + 2] : 1.0;
         for(int k = 0; k < num_steps; k++) {
-            if(model == ORC_TT2006) orc_tt2006_step(y, st, dt);
+            if(model == ORC_TT2006) orc_tt2006_step_scaled(y, st, dt, s0, s1, s2);
             else if(model == ORC_ORD2011) orc_ord2011_step(y, st, dt);
-            else ms_step(y, st, dt);
+            else ms_step(y, st, dt, s0);
         }
     }
+}
+
+void orc_model_solve(int model, double *sv, int64_t n, const double *stim, double dt, int num_steps) {
+    orc_model_solve_scaled(model, sv, n, stim, 0, dt, num_steps);
 }

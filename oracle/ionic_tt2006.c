@@ -24,8 +24,10 @@ void orc_tt2006_init_cell(double *y) {
     for(int s = 0; s < TT_NEQ; s++) y[s] = (double)ic[s];
 }
 
-/* One step: y (in/out), stim in uA/uF (subtracted from dV/dt). */
-void orc_tt2006_step(double *y, double stim, double dt) {
+/* One step: y (in/out), stim in uA/uF (subtracted from dV/dt).  sKr, sCaL and
+ * sKs multiply g_Kr, g_CaL and g_Ks (batch sweeps; no reference counterpart,
+ * all 1 is the reference model: x * 1.0 is exact). */
+void orc_tt2006_step_scaled(double *y, double stim, double dt, double sKr, double sCaL, double sKs) {
     /* state */
     const double V = y[0], Xr1 = y[1], Xr2 = y[2], Xs = y[3], m = y[4], h = y[5], j = y[6];
     const double d = y[7], f = y[8], f2 = y[9], fCass = y[10], s = y[11], r = y[12];
@@ -39,7 +41,7 @@ void orc_tt2006_step(double *y, double stim, double dt) {
     const double Vmax_up = 0.006375;
     const double nx_alpha = 2.5, nx_gamma = 0.35, K_sat = 0.1, Km_Ca = 1.38, Km_Nai = 87.5;
     const double K_NaCa = 1000.0;
-    const double g_to = 0.294, g_Kr = 0.153, g_Ks = 0.098, g_CaL = 3.98e-05, g_Na = 14.838;
+    const double g_to = 0.294, g_Kr = 0.153 * sKr, g_Ks = 0.098 * sKs, g_CaL = 3.98e-05 * sCaL, g_Na = 14.838;
     const double g_pK = 0.0146, g_bca = 0.000592, g_pCa = 0.1238, g_K1 = 5.405, g_bna = 0.00029;
 
     /* reversal potentials (:206-209) */
@@ -183,3 +185,5 @@ void orc_tt2006_step(double *y, double stim, double dt) {
     y[17] = dt * dNai + Na_i;
     y[18] = dt * dKi + K_i;
 }
+
+void orc_tt2006_step(double *y, double stim, double dt) { orc_tt2006_step_scaled(y, stim, dt, 1.0, 1.0, 1.0); }

@@ -32,6 +32,10 @@ void   orc_model_init(int model, double *sv, int64_t n);
 /* num_steps explicit steps of size dt for n cells, per-cell stimulus (AoS) */
 void   orc_model_solve(int model, double *sv, int64_t n, const double *stim,
                        double dt, int num_steps);
+/* the same with per-cell model scalings, 4 floats per cell (TT2006: GKr, GCaL,
+ * GKs; MS2003: tau_close; ignored for ORd); scale == NULL is all 1 */
+void   orc_model_solve_scaled(int model, double *sv, int64_t n, const double *stim,
+                              const float *scale, double dt, int num_steps);
 
 /* ALPHA(beta, cm, dt, dx, dy, dz) of monodomain/constants.h:10-11 */
 double orc_alpha(double beta, double cm, double dt, double dx, double dy, double dz);
@@ -100,6 +104,11 @@ struct orc_tissue {
     int64_t total_cg_its;
     int last_cg_its;
     double last_cg_err;
+    /* optional (no reference counterpart; batch sweeps): per-cell 0/1 stimulated
+     * cells replacing the x < stim_x_limit rule, and per-cell model scalings
+     * (4 floats per cell, see orc_model_solve_scaled) */
+    const uint8_t *stim_mask;
+    const float *ionic_scale;
 };
 
 /* set_initial_conditions_fvm (matrix_assembly.c:27-44) + model ICs */

@@ -42,6 +42,7 @@ def lib() -> C.CDLL:
         L.orc_model_initial_v.argtypes = [C.c_int]
         L.orc_model_init.argtypes = [C.c_int, _dp, C.c_int64]
         L.orc_model_solve.argtypes = [C.c_int, _dp, C.c_int64, C.c_void_p, C.c_double, C.c_int]
+        L.orc_model_solve_scaled.argtypes = [C.c_int, _dp, C.c_int64, C.c_void_p, C.c_void_p, C.c_double, C.c_int]
         L.orc_alpha.restype = C.c_double
         L.orc_alpha.argtypes = [C.c_double] * 6
         L.orc_assemble_iso.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [_dp, _u8p, _dp]
@@ -89,6 +90,18 @@ def model_solve(model: int, sv: np.ndarray, stim: np.ndarray | None, dt: float, 
     n = sv.shape[0]
     st = None if stim is None else np.ascontiguousarray(stim, dtype=np.float64)
     lib().orc_model_solve(model, sv.reshape(-1), n, None if st is None else st.ctypes.data, dt, num_steps)
+
+
+def model_solve_scaled(model: int, sv: np.ndarray, stim: np.ndarray | None, scale: np.ndarray, dt: float,
+                       num_steps: int = 1) -> None:
+    """model_solve with per-cell scalings scale (n, 4) float32: TT2006 GKr, GCaL,
+    GKs; MS2003 tau_close (no reference counterpart; all 1 is model_solve)."""
+    assert sv.dtype == np.float64 and sv.flags.c_contiguous
+    n = sv.shape[0]
+    sc = np.ascontiguousarray(scale, dtype=np.float32).reshape(n, 4)
+    st = None if stim is None else np.ascontiguousarray(stim, dtype=np.float64)
+    lib().orc_model_solve_scaled(model, sv.reshape(-1), n, None if st is None else st.ctypes.data, sc.ctypes.data, dt,
+                                 num_steps)
 
 
 def alpha(beta, cm, dt, dx, dy, dz) -> float:
@@ -166,14 +179,15 @@ class _Tissue(C.Structure):
                 ("stim_x_limit", C.c_double), ("stim_period", C.c_double), ("coef27", C.c_void_p),
                 ("active", C.c_void_p), ("sv", C.c_void_p), ("v", C.c_void_p), ("b", C.c_void_p),
                 ("cur_time", C.c_double), ("count", C.c_int64), ("total_cg_its", C.c_int64),
-                ("last_cg_its", C.c_int), ("last_cg_err", C.c_double)]
+                ("last_cg_its", C.c_int), ("last_cg_err", C.c_double), ("stim_mask", C.c_void_p),
+                ("ionic_scale", C.c_void_p)]
 
 
 class Tissue:
     """The oracle's operator-split run on a structured box (AoS state)."""
 
     def __init__(self, shape, h, model, coef27, dt=0.02, beta=0.14, cm=1.0, cg_tol=1e-16, cg_max_its=200,
-                 jacobi=False, stim=(0.0, 2.0, -50.0, 500.0, 0.0), active=None):
+                 jacobi=False, stim=(0.0, 2.0, -50.0, 500.0, 0.0), active=None, stim_mask=None, ionic_scale=None):
         nx, ny, nz = shape
         n = nx * ny * nz
         self.shape, self.n, self.model = shape, n, model
@@ -192,6 +206,10 @@ class Tissue:
         t.coef27 = self.coef27.ctypes.data
         t.active = None if self.active is None else self.active.ctypes.data
         t.sv, t.v, t.b = self.sv.ctypes.data, self.v.ctypes.data, self.b.ctypes.data
+        self.stim_mask = None if stim_mask is None else np.ascontiguousarray(stim_mask, np.uint8)
+        self.ionic_scale = None if ionic_scale is None else np.ascontiguousarray(ionic_scale, np.float32).reshape(n, 4)
+        t.stim_mask = None if self.stim_mask is None else self.stim_mask.ctypes.data
+        t.ionic_scale = None if self.ionic_scale is None else self.ionic_scale.ctypes.data
         self._t = t
         lib().orc_tissue_init(C.byref(t))
 

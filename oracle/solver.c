@@ -193,16 +193,18 @@ void orc_tissue_step(struct orc_tissue *t) {
         for(int64_t i = 0; i < n; i++) {
             const int xi = (int)(i % t->nx);
             const double cx = xi * t->dx + t->dx / 2.0;
-            if(cx < t->stim_x_limit) stim[i] += t->stim_current;
+            if(t->stim_mask ? t->stim_mask[i] != 0 : cx < t->stim_x_limit) stim[i] += t->stim_current;
         }
     }
     if(t->stim_period > 0.0 && time + t->dt >= t->stim_start + t->stim_period) t->stim_start += t->stim_period;
     /* ODE for the active cells */
     if(!t->active) {
-        orc_model_solve(t->model, t->sv, n, stim, t->dt, 1);
+        orc_model_solve_scaled(t->model, t->sv, n, stim, t->ionic_scale, t->dt, 1);
     } else {
         for(int64_t i = 0; i < n; i++)
-            if(t->active[i]) orc_model_solve(t->model, t->sv + i * neq, 1, stim + i, t->dt, 1);
+            if(t->active[i])
+                orc_model_solve_scaled(t->model, t->sv + i * neq, 1, stim + i,
+                                       t->ionic_scale ? t->ionic_scale + 4 * i : 0, t->dt, 1);
     }
     free(stim);
     /* update_monodomain_default (update_monodomain.c:19-56) */

@@ -208,13 +208,20 @@ def lat_from(snaps, every, thr=-30.0):
     return lat
 
 
-def tissue(tmp, name, shape, model, T, every, aniso=False, fibrosis=0.0, keep_times=()):
+def tissue(tmp, name, shape, model, T, every, aniso=False, fibrosis=0.0, keep_times=(), cable=False):
     nx, ny, nz = shape
     n = nx * ny * nz
     out_dir = os.path.join(tmp, name)
     os.makedirs(out_dir, exist_ok=True)
     active = np.ones(n, np.uint8)
-    if not aniso:
+    if cable:
+        assert ny == 1 and nz == 1 and not aniso
+        domain = (f"name=Cable Mesh\nstart_dx={H}\nstart_dy={H}\nstart_dz={H}\ncable_length={nx * H}\n"
+                  f"main_function=initialize_grid_with_cable_mesh")
+        asm = ("sigma_x=0.00013\nsigma_y=0.00013\nsigma_z=0.00013\n"
+               "main_function=homogeneous_sigma_assembly_matrix")
+        fib = None
+    elif not aniso:
         domain = (f"name=Cuboid Mesh\nstart_dx={H}\nstart_dy={H}\nstart_dz={H}\nside_length_x={nx * H}\n"
                   f"side_length_y={ny * H}\nside_length_z={nz * H}\nmain_function=initialize_grid_with_cuboid_mesh")
         asm = ("sigma_x=0.00013\nsigma_y=0.00013\nsigma_z=0.00013\n"
@@ -256,7 +263,8 @@ def tissue(tmp, name, shape, model, T, every, aniso=False, fibrosis=0.0, keep_ti
     if fib is not None:
         fx["fibres"] = fib
     np.savez_compressed(os.path.join(HERE, f"tissue_{name}.npz"), **fx)
-    PROV[f"tissue_{name}.npz"] = (f"reference MonoAlg3D (oracle/_ref) {nx}x{ny}x{nz} cells at {H} um, {model}, "
+    PROV[f"tissue_{name}.npz"] = (f"reference MonoAlg3D (oracle/_ref) {nx}x{ny}x{nz} cells at {H} um"
+                                  f"{' (cable mesh)' if cable else ''}, {model}, "
                                   f"{'anisotropic 0.24/0.045 S/m, fibres -60..60 deg over z (custom mesh + fibres file)' if aniso else 'isotropic 0.13 S/m'}"
                                   f"{f', {fibrosis:.0%} cells omitted (inactive)' if fibrosis else ''}; stimulus -50 on x<500 um for "
                                   f"t in [0,2] ms; CG tol 1e-16; {T} ms; Vm saved every {every} steps; LAT = first crossing of "
@@ -351,7 +359,7 @@ def main():
     prov_path = os.path.join(HERE, "provenance.json")
     if os.path.exists(prov_path):
         PROV.update(json.load(open(prov_path)))
-    tmp = tempfile.mkdtemp(prefix="golden_", dir=os.path.join(ROOT, "_work") if os.path.isdir(os.path.join(ROOT, "_work")) else None)
+    tmp = tempfile.mkdtemp(prefix="golden_")
     try:
         if "ionic" in groups:
             ionic(tmp)
@@ -365,6 +373,10 @@ def main():
             solvers(tmp)
         if "activation" in groups:
             activation(tmp)
+        if "batch" in groups:
+            # instances of a batch, each run on its own by the reference
+            tissue(tmp, "cable_tt2006_300", (300, 1, 1), "tt2006", 60.0, 50, cable=True, keep_times=(20, 60))
+            tissue(tmp, "sheet_tt2006_48x32", (48, 32, 1), "tt2006", 40.0, 50, keep_times=(20, 40))
         if "config1" in groups:
             tissue(tmp, "config1_tt2006_64x64x16", (64, 64, 16), "tt2006", 50.0, 10, keep_times=(20,))
     finally:

@@ -1,0 +1,134 @@
+"""Batched runs on the GPU: many instances packed into one atlas.
+
+Each instance of a batch must equal its own run: the reference's own cable and
+sheet runs (golden fixtures written by MonoAlg3D built from its sources) for
+instances with the reference parameters, the oracle's packed run for scaled
+instances, and the GPU's own stand-alone runs.  A batch's CG stops on the
+atlas residual, a stand-alone run on its own, so each step's solves may differ
+by |r|/alpha <= 1e-8/0.07 = 1.4e-7 mV; over the 1,000-3,000 steps of these runs
+that allows up to 4e-4 mV, and FP64 runs agree within VM_TOL = 1e-4 mV (the
+wave front of a 60 ms cable run shows 2e-5 mV) with activation times within
+1e-3 ms.  FP32 + refinement keeps activation times within 0.5 ms of the FP64
+batch.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+sys.path.insert(0, GOLD)
+from make_golden import lat_from  # noqa: E402
+
+H = (100.0, 100.0, 100.0)
+VM_TOL = 1e-4
+
+
+def _mixed():
+    from monoalg3d_c_amd.batch import Instance
+    return [Instance(300, 1), Instance(48, 32), Instance(150, 1, (1.3, 0.7, 1.2, 1.0)),
+            Instance(20, 20, (0.6, 1.4, 0.8, 1.0), stim_x_limit=300.0), Instance(64, 1, (0.5, 1.5, 1.5, 1.0)),
+            Instance(9, 40)]
+
+
+def _snapshots(b, every, nsamp):
+    out = [b.v()]
+    for _ in range(nsamp - 1):
+        b.step(every)
+        out.append(b.v())
+    return [np.array([s[i] for s in out]) for i in range(len(out[0]))]
+
+
+def test_batch_instances_match_reference_runs():
+    """A 300-cell cable and a 48x32 sheet with the reference parameters, packed
+    with four scaled instances, against MonoAlg3D's stand-alone runs of each."""
+    import monoalg3d_c_amd as m
+    from monoalg3d_c_amd.batch import Batch
+    announce("batch fp64 vs reference cable_tt2006_300 + sheet_tt2006_48x32, 60 ms")
+    b = Batch(_mixed(), precision=m.FP64, cg_tol=1e-16, tensor=m.iso_tensor(0.00013))
+    assert b.layout.shape[2] == 1
+    snaps = _snapshots(b, 50, 61)
+    st = b.stats
+    b.close()
+    assert st["nan_detected"] == 0
+    for i, name in ((0, "tissue_cable_tt2006_300"), (1, "tissue_sheet_tt2006_48x32")):
+        g = np.load(os.path.join(GOLD, name + ".npz"))
+        ns = len(g["probe_vm"])
+        s = snaps[i][:ns]
+        dv = np.abs(s[:, g["probe_cells"]] - g["probe_vm"]).max()
+        assert dv < VM_TOL, (name, dv)
+        for key in g.files:
+            if key.startswith("vm_t"):
+                k = int(round(int(key[4:]) / (50 * 0.02)))
+                assert np.abs(s[k] - g[key]).max() < VM_TOL, (name, key)
+        lat = lat_from(s, 50)
+        assert np.array_equal(np.isfinite(lat), np.isfinite(g["lat"])), name
+        assert np.abs(lat - g["lat"]).max() <= 1e-3, name
+
+
+def test_batch_matches_oracle_atlas(oracle):
+    """Scaled instances (per-instance GKr/GCaL/GKs and stimulus region): the
+    GPU atlas against the oracle's run of the same atlas, every cell, 30 ms."""
+    import monoalg3d_c_amd as m
+    from monoalg3d_c_amd.batch import Batch, Layout
+    insts = _mixed()[2:]
+    L = Layout(insts)
+    announce(f"batch fp64 vs oracle atlas {L.shape}, 30 ms")
+    n = L.n
+    a = oracle.alpha(0.14, 1.0, 0.02, *H)
+    coef = oracle.assemble_iso(L.shape, H, a, np.full((n, 3), 0.00013), L.active())
+    t = oracle.Tissue(L.shape, H, oracle.MODEL_IDS["tt2006"], coef, active=L.active(), stim_mask=L.stim_mask(100.0),
+                      ionic_scale=L.ionic_scale())
+    b = Batch(insts, precision=m.FP64, cg_tol=1e-16, tensor=m.iso_tensor(0.00013))
+    for _ in range(3):
+        t.step(500)
+        b.step(500)
+        got = b.v()
+        for i, want in enumerate(L.split(t.v)):
+            np.testing.assert_allclose(got[i], want, rtol=0, atol=VM_TOL, err_msg=f"instance {i}")
+    b.close()
+
+
+def test_batch_instance_equals_standalone_gpu_run():
+    """Each scaled instance of the batch against a stand-alone GPU tissue of
+    its own shape with its scalings on every cell (FP64, 20 ms)."""
+    import monoalg3d_c_amd as m
+    from monoalg3d_c_amd.batch import Batch
+    insts = _mixed()
+    announce("batch fp64 vs standalone gpu runs, 20 ms")
+    b = Batch(insts, precision=m.FP64, cg_tol=1e-16, tensor=m.iso_tensor(0.00013))
+    b.step(1000)
+    got = b.v()
+    b.close()
+    for i, inst in enumerate(insts):
+        sm = ((np.arange(inst.nx) + 0.5) * 100.0 < inst.stim_x_limit).astype(np.uint8)
+        t = m.Tissue((inst.nx, inst.ny, 1), precision=m.FP64, cg_tol=1e-16, tensor=m.iso_tensor(0.00013),
+                     stim_mask=np.tile(sm, inst.ny), ionic_scale=np.tile(np.asarray(inst.scale, np.float32), (inst.n, 1)))
+        t.step(1000)
+        np.testing.assert_allclose(got[i], t.v, rtol=0, atol=VM_TOL, err_msg=f"instance {i}")
+        t.close()
+
+
+def test_batch_fp32_refine_activation_times():
+    """FP32 states + FP32 CG with FP64 defect correction: every instance's
+    activation times within 0.5 ms of the FP64 batch, same cells activated."""
+    import monoalg3d_c_amd as m
+    from monoalg3d_c_amd.batch import Batch
+    announce("batch fp32+refine vs fp64 activation times, 60 ms")
+    res = {}
+    for prec in (m.FP64, m.FP32_REFINE):
+        b = Batch(_mixed(), precision=prec, cg_tol=1e-16, tensor=m.iso_tensor(0.00013))
+        res[prec] = _snapshots(b, 10, 301)
+        assert b.stats["nan_detected"] == 0
+        b.close()
+    for i in range(len(_mixed())):
+        l64, l32 = lat_from(res[m.FP64][i], 10), lat_from(res[m.FP32_REFINE][i], 10)
+        assert np.isfinite(l64).any()
+        assert np.array_equal(np.isfinite(l64), np.isfinite(l32)), i
+        fin = np.isfinite(l64)
+        assert np.abs(l64[fin] - l32[fin]).max() <= 0.5, i

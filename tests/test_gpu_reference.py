@@ -70,6 +70,18 @@ def _check(g, snaps, st, vm_tol, lat_tol, vm_times):
     return dv, dlat
 
 
+@pytest.mark.parametrize("name,times", [("tissue_cable_tt2006_300", (20, 60)), ("tissue_sheet_tt2006_48x32", (20, 40))])
+def test_cable_and_sheet_fp64_match_reference(name, times):
+    """One-cell-thick domains: the reference's cable mesh (300 x 1 x 1, 60 ms)
+    and a 48 x 32 x 1 sheet (40 ms).  Over the 3,000 steps of the cable the
+    per-step solver differences (|r|/alpha <= 1.4e-7 mV) may add up to 4e-4 mV:
+    Vm within 1e-4 mV, activation times within 1e-3 ms."""
+    import monoalg3d_c_amd as m
+    announce(f"reference {name} fp64")
+    g, snaps, st = _run(name, "tt2006", m.FP64)
+    _check(g, snaps, st, 1e-4, 1e-3, times)
+
+
 def test_config1_fp64_matches_reference():
     """64x64x16 TT2006, isotropic 0.13 S/m, planar stimulus, 50 ms (2,501 steps)."""
     import monoalg3d_c_amd as m

@@ -85,6 +85,80 @@ def test_ionic_rest_is_steady(oracle):
         assert abs(vs[-1] - vs[99]) < 0.5, (name, vs[99], vs[-1])
 
 
+# ------------------------------------------------- ionic scalings --------
+# Per-cell scalings have no reference counterpart (batch sweeps scale TT2006
+# GKr/GCaL/GKs and MS2003 tau_close).  All ones must be the reference model
+# bit for bit; otherwise each factor is pinned by what it must do to the
+# model's equations: act only through its own current (zeroing that current's
+# gate removes the factor's effect, zeroing another gate does not), enter the
+# Euler step of V affinely with the physiological sign, and for tau_close
+# divide the closing rate of h only above V_gate.
+
+@pytest.mark.parametrize("name", ["tt2006", "ord2011", "ms2003"])
+def test_scaling_ones_is_reference_model(oracle, name):
+    g = _load(f"ionic_{name}")
+    mid = oracle.MODEL_IDS[name]
+    sv = np.ascontiguousarray(g["pair_in"])
+    oracle.model_solve_scaled(mid, sv, g["pair_stim"], np.ones((len(sv), 4), np.float32), float(g["dt"]), 1)
+    np.testing.assert_array_equal(sv, g["pair_out"])
+
+
+def _tt_plateau(oracle):
+    mid = oracle.MODEL_IDS["tt2006"]
+    sv = oracle.model_init(mid, 1)
+    on = _stim_on(101)
+    for k in range(101):
+        oracle.model_solve(mid, sv, np.array([-50.0 if on[k] else 0.0]), DT, 1)
+    oracle.model_solve(mid, sv, np.zeros(1), DT, 2000)
+    assert sv[0, 0] > 0.0, sv[0, 0]  # 40 ms after the stimulus: plateau, outward IKr/IKs, inward ICaL
+    return sv
+
+
+# scale slot -> (state index of its current's gate, sign of dV per unit factor)
+TT_SLOTS = {0: (1, -1), 1: (7, +1), 2: (3, -1)}  # GKr via Xr1, GCaL via d, GKs via Xs
+
+
+def test_tt2006_scaling_acts_on_its_own_current(oracle):
+    mid = oracle.MODEL_IDS["tt2006"]
+    base = _tt_plateau(oracle)
+    for slot, (gate, sign) in TT_SLOTS.items():
+        for zeroed, _ in TT_SLOTS.values():
+            y = base.copy()
+            y[0, zeroed] = 0.0
+            outs = []
+            for f in (0.5, 1.0, 1.5):
+                sc = np.ones((1, 4), np.float32)
+                sc[0, slot] = f
+                z = y.copy()
+                oracle.model_solve_scaled(mid, z, np.zeros(1), sc, DT, 1)
+                outs.append(z[0])
+            if zeroed == gate:
+                np.testing.assert_array_equal(outs[0], outs[1])
+                np.testing.assert_array_equal(outs[2], outs[1])
+            else:
+                lo, mid_v, hi = outs[0][0], outs[1][0], outs[2][0]
+                assert np.sign(hi - mid_v) == sign and np.sign(mid_v - lo) == sign, (slot, lo, mid_v, hi)
+                assert abs((hi - mid_v) - (mid_v - lo)) <= 1e-9 * abs(hi - mid_v), (slot, lo, mid_v, hi)
+
+
+def test_ms2003_scaling_divides_closing_rate(oracle):
+    mid = oracle.MODEL_IDS["ms2003"]
+    for V, closes in ((0.5, True), (0.05, False)):
+        steps = []
+        for f in (0.5, 1.0, 2.0):
+            y = np.array([[V, 0.8]])
+            sc = np.array([[f, 1, 1, 1]], np.float32)
+            oracle.model_solve_scaled(mid, y, np.zeros(1), sc, DT, 1)
+            steps.append((y[0, 1] - 0.8, f, y[0, 0]))
+        assert steps[0][2] == steps[1][2] == steps[2][2]  # V does not see tau_close
+        if closes:
+            # h' = -h / (150 s): the step times the factor is constant and equals -dt h / 150
+            for dh, f, _ in steps:
+                assert abs(dh * f - (-DT * 0.8 / 150.0)) < 1e-15
+        else:
+            assert steps[0][0] == steps[1][0] == steps[2][0] > 0.0
+
+
 # ------------------------------------------------------------- assembly ----
 
 def _alpha(oracle):
@@ -257,6 +331,8 @@ def _tissue_case(oracle, name, model, aniso):
 @pytest.mark.parametrize("name,model,aniso", [
     ("tissue_iso_tt2006_32x32x8", "tt2006", False),
     ("tissue_fibrotic_ord_24x24x6", "ord2011", True),
+    ("tissue_cable_tt2006_300", "tt2006", False),
+    ("tissue_sheet_tt2006_48x32", "tt2006", False),
 ])
 def test_tissue_run_matches_reference(oracle, name, model, aniso):
     """The oracle's operator-split loop against MonoAlg3D's own run: probe Vm
