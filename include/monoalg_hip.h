@@ -115,7 +115,8 @@ typedef struct mono_stats {
     int64_t inner_iterations_total; /* CG iterations actually run (all sweeps) */
     int64_t outer_sweeps_total;
     int nan_detected;               /* the reference stops on a NaN solver error */
-    int class_operator;             /* 1: operator stored as per-layer class table */
+    int class_operator;             /* operator storage: 0 per-cell rows, 1 per-layer class table,
+                                       2 neighbour-code table (one byte per cell) */
     int stencil_points;             /* stored non-zeros per row */
     double ionic_ms_total;          /* device time in the ionic kernel (timing on) */
     double solve_ms_total;          /* device time in the diffusion solve (timing on) */

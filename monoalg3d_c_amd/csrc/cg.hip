@@ -193,6 +193,7 @@ template <typename TC> __device__ __forceinline__ double diag_of(const OpView<TC
         const int yi = (int)(t % op.ny), zi = (int)(t / op.ny);
         return (double)op.table[((zi * 3 + pos_cls(yi, op.ny)) * 3 + pos_cls(xi, op.nx)) * 27 + 13];
     }
+    if(op.codes) return (double)op.ctab[(int)op.codes[c] * op.pat];
     return (double)op.percell[c];
 }
 
@@ -265,13 +266,22 @@ __device__ __forceinline__ TV row27_lds(const TC *row, const TV *const (&P3)[3],
 }
 
 // Planes z0..z1-1 of xy tile t.
-template <typename TV, typename TC, bool CLS, int PAT, bool FUSE, class Epi>
+template <typename TV, typename TC, int CM, int PAT, bool FUSE, class Epi>
 __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeom &g, int t, int z0, int z1, const TV *__restrict__ x,
                                              const TV *__restrict__ rz, TV beta, TV *__restrict__ p_out, const Epi &epi,
                                              double (&v)[Epi::NR]) {
     static_assert(9 * 27 <= kBlock, "one class-row coefficient per thread");
+    // CM: 0 per-cell rows, 1 class table, 2 code table (7-point rows)
+    constexpr bool CLS = CM == 1;
+    static_assert(CM != 2 || PAT == 7, "code tables hold 7-point rows");
     __shared__ TV pl[3 * kPL];
     __shared__ TC crows[CLS ? 9 * 27 : 1];
+    __shared__ TC ctab[CM == 2 ? kCodes * PAT : 1];
+    if constexpr(CM == 2) {
+        // the code table (identical in every segment: rewriting it is benign);
+        // the first barrier of the plane loop orders it before any use
+        for(int i = threadIdx.x; i < kCodes * PAT; i += kBlock) ctab[i] = op.ctab[i];
+    }
     const int x0 = (t % g.ntx) * kTX, y0 = (t / g.ntx) * kTY;
     const int tid = threadIdx.x;
     const int64_t sz = (int64_t)op.nx * op.ny;
@@ -355,7 +365,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
             const int base = (rr + 1) * kW + tx + 1;
             TV acc;
             double dg = 0.0;  // the row's diagonal (Jacobi epilogues only)
-            if constexpr(!CLS) {
+            if constexpr(CM == 0) {
                 // slot arrays start at uniform addresses; a 32-bit cell offset
                 // lets every load use a scalar base (callers keep n <= 2^31)
                 const uint32_t cu = (uint32_t)c;
@@ -365,6 +375,13 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
                     acc += (TV)(op.percell + (int64_t)k * op.n)[cu] *
                            P3[RowPattern<PAT>::o[k][2] + 1][base + RowPattern<PAT>::o[k][1] * kW + RowPattern<PAT>::o[k][0]];
                 if constexpr(Epi::kDiag) dg = (double)op.percell[cu];
+            } else if constexpr(CM == 2) {
+                const TC *row = ctab + (int)op.codes[c] * PAT;
+                acc = TV(0);
+#pragma unroll
+                for(int k = 0; k < PAT; k++)
+                    acc += (TV)row[k] * P3[RowPattern<PAT>::o[k][2] + 1][base + RowPattern<PAT>::o[k][1] * kW + RowPattern<PAT>::o[k][0]];
+                if constexpr(Epi::kDiag) dg = (double)row[0];
             } else {
                 if(!xf) continue;
                 if(zf && gy >= g.lo[1] && gy <= g.hi[1]) {
@@ -434,7 +451,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
     }
 }
 
-template <typename TV, typename TC, bool CLS, int PAT, bool FUSE, class Epi>
+template <typename TV, typename TC, int CM, int PAT, bool FUSE, class Epi>
 __device__ __forceinline__ void tile_sweep(const OpView<TC> &op, const TileGeom &g, const TV *__restrict__ x, const TV *__restrict__ rz,
                                            TV beta, TV *__restrict__ p_out, const Epi &epi, double (&v)[Epi::NR]) {
     // runs of one XCD's blocks are adjacent (shared halos stay in its L2)
@@ -444,7 +461,7 @@ __device__ __forceinline__ void tile_sweep(const OpView<TC> &op, const TileGeom 
     while(w < end) {
         const int t = (int)(w / op.nz), z0 = (int)(w % op.nz);
         const int z1 = (int64_t)op.nz - z0 < end - w ? op.nz : z0 + (int)(end - w);
-        tile_segment<TV, TC, CLS, PAT, FUSE, Epi>(op, g, t, z0, z1, x, rz, beta, p_out, epi, v);
+        tile_segment<TV, TC, CM, PAT, FUSE, Epi>(op, g, t, z0, z1, x, rz, beta, p_out, epi, v);
         w += z1 - z0;
     }
 }
@@ -460,7 +477,7 @@ __device__ __forceinline__ void residency_probe(int spin, double *out) {
 }
 
 // Residual kernels: no prologue.  Iteration kernel (q = A p): prologue first.
-template <typename TV, typename TC, bool CLS, int PAT, bool FUSE, class Epi>
+template <typename TV, typename TC, int CM, int PAT, bool FUSE, class Epi>
 __global__ __launch_bounds__(kBlock) void k_stencil(OpView<TC> op, TileGeom g, const TV *__restrict__ x, const TV *__restrict__ rz,
                                                     TV *__restrict__ p_out, Epi epi, CgSetup cs, int mode, int k, const double *Pin,
                                                     int nprev, double *Pout, CgScalars *sc, CgScalars *so) {
@@ -474,7 +491,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(OpView<TC> op, TileGeom g, c
     double v[Epi::NR];
 #pragma unroll
     for(int j = 0; j < Epi::NR; j++) v[j] = 0.0;
-    tile_sweep<TV, TC, CLS, PAT, FUSE, Epi>(op, g, x, rz, beta, p_out, epi, v);
+    tile_sweep<TV, TC, CM, PAT, FUSE, Epi>(op, g, x, rz, beta, p_out, epi, v);
     write_partials<Epi::NR>(v, Pout);
 }
 
@@ -721,11 +738,15 @@ template <class F> static void with_pat(int P, F &&f) {
 }
 
 // Call f(IC<S>, IC<PAT>) for the operator's class-row slot count and row pattern.
-// Call f(IC<CLS>, IC<PAT>) for the operator's storage mode and row pattern.
+// Call f(IC<CM>, IC<PAT>) for the operator's storage mode (0 per-cell rows,
+// 1 class table, 2 code table) and row pattern.
 template <typename TC, class F> static void with_op(const OpView<TC> &op, F &&f) {
     with_pat(op.pat, [&](auto P_) {
         if(op.table) f(IC<1>{}, P_);
-        else f(IC<0>{}, P_);
+        else if constexpr(decltype(P_)::value == 7) {
+            if(op.codes) f(IC<2>{}, P_);
+            else f(IC<0>{}, P_);
+        } else f(IC<0>{}, P_);
     });
 }
 
@@ -831,7 +852,7 @@ static int poll(CgScalars *dev, CgScalars *host, hipStream_t s) {
 template <typename TV, typename TC, int C, int P, bool FUSE, class Epi>
 static int launch_stencil(const OpView<TC> &op, const TV *x, const TV *rz, TV *p_out, const Epi &epi, const CgSetup &cs, int mode,
                           int k, const double *Pin, int nprev, double *Pout, CgScalars *sc, CgScalars *so, hipStream_t s) {
-    constexpr auto K = &k_stencil<TV, TC, C != 0, P, FUSE, Epi>;
+    constexpr auto K = &k_stencil<TV, TC, C, P, FUSE, Epi>;
     static int slots[64] = {};
     int dev = 0;
     if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;

@@ -73,8 +73,18 @@ template <typename T> struct OpView {
     const T *table;             // class mode: [(z*9 + cy*3 + cx) * 27 + slot], else null
     const T *table_int;         // class mode: [z * pat + k] interior rows, else null
     const T *percell;           // per-cell mode: [k * n + cell], else null
+    const uint8_t *codes;       // code mode: [cell] row code (7-point rows only), else null
+    const T *ctab;              // code mode: [code * pat + k], kCodes rows, else null
     const uint8_t *active;      // optional
 };
+
+// Code mode: when every 7-point row is fixed by which of its six neighbours
+// exist and are active (a uniform isotropic tensor on a box with inactive
+// cells, e.g. many instances packed with gaps, or holes), the row of a cell is
+// one of kCodes table rows, picked by a one-byte code: bit 0 the cell's own
+// activity, bit k (k = 1..6) neighbour k of RowPattern<7>.  One byte per cell
+// replaces 7 FP64 / FP32 coefficients.
+constexpr int kCodes = 128;
 
 struct Operator {
     int nx = 0, ny = 0, nz = 0;
@@ -83,6 +93,10 @@ struct Operator {
     int nint = 0;          // class mode: distinct slots of the interior rows
     int pat = 27;          // pattern of the interior rows (class mode) or of all rows (per-cell)
     bool class_mode = false;
+    bool code_mode = false;
+    uint8_t *codes = nullptr;
+    double *ctab64 = nullptr;
+    float *ctab32 = nullptr;
     double *table64 = nullptr, *tint64 = nullptr;
     float *table32 = nullptr, *tint32 = nullptr;
     double *percell64 = nullptr;

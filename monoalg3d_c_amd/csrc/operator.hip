@@ -392,6 +392,126 @@ __global__ __launch_bounds__(256) void k_slot_mask(const double *coef27, int64_t
     if((threadIdx.x & 63) == 0 && m) atomicOr(mask, m);
 }
 
+// Row code of each cell (see kCodes): own activity and, for each 7-point
+// neighbour, whether it exists in the box and is active.
+__global__ __launch_bounds__(256) void k_row_codes(int nx, int ny, int nz, const uint8_t *active, uint8_t *codes) {
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(c >= n) return;
+    const int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / ((int64_t)nx * ny));
+    auto on = [&](int xx, int yy, int zz) -> unsigned {
+        if(xx < 0 || xx >= nx || yy < 0 || yy >= ny || zz < 0 || zz >= nz) return 0u;
+        return !active || active[xx + (int64_t)nx * (yy + (int64_t)ny * zz)] ? 1u : 0u;
+    };
+    unsigned code = on(x, y, z);
+    if(code) {
+#pragma unroll
+        for(int k = 1; k < 7; k++) {
+            const int8_t *o = RowPattern<7>::o[k];
+            code |= on(x + o[0], y + o[1], z + o[2]) << k;
+        }
+    }
+    codes[c] = (uint8_t)code;
+}
+
+// first cell of each code (reps start at all ones = absent)
+__global__ __launch_bounds__(256) void k_code_reps(const uint8_t *codes, int64_t n, unsigned long long *reps) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(c >= n) return;
+    const unsigned code = codes[c];
+    if((unsigned long long)c < reps[code]) atomicMin(&reps[code], (unsigned long long)c);
+}
+
+__global__ __launch_bounds__(256) void k_verify_codes(const double *coef27, int64_t n, const uint8_t *codes, const double *table27,
+                                                      unsigned long long *mismatch) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(c >= n) return;
+    const int code = codes[c];
+    bool bad = false;
+    for(int k = 0; k < 27; k++) {
+        const double a = coef27[(int64_t)k * n + c], b = table27[code * 27 + k];
+        if(__double_as_longlong(a) != __double_as_longlong(b)) bad = true;
+    }
+    if(bad) atomicAdd(mismatch, 1ull);
+}
+
+// Try code mode for a 7-point operator; returns 1 when every row matched its
+// code's table row (op->codes and the tables set), 0 when not, -1 on error.
+static int try_code_mode(const AssemblyParams &p, const double *coef27, const std::vector<int> &pslots, Operator *op, hipStream_t s) {
+    const int64_t n = (int64_t)p.nx * p.ny * p.nz;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    uint8_t *codes = nullptr;
+    unsigned long long *dreps = nullptr, *dmis = nullptr;
+    int64_t *drep = nullptr;
+    double *dtable27 = nullptr;
+    auto cleanup = [&]() {
+        void *bufs[] = {codes, dreps, dmis, drep, dtable27};
+        for(void *b : bufs)
+            if(b) (void)hipFree(b);
+    };
+    if(hipMalloc(&codes, n) != hipSuccess || hipMalloc(&dreps, sizeof(unsigned long long) * kCodes) != hipSuccess ||
+       hipMalloc(&dmis, sizeof(unsigned long long)) != hipSuccess || hipMalloc(&drep, sizeof(int64_t) * kCodes) != hipSuccess ||
+       hipMalloc(&dtable27, sizeof(double) * kCodes * 27) != hipSuccess) {
+        cleanup();
+        mono_set_error("out of device memory building the operator");
+        return -1;
+    }
+    k_row_codes<<<grid, 256, 0, s>>>(p.nx, p.ny, p.nz, p.active, codes);
+    (void)hipMemsetAsync(dreps, 0xff, sizeof(unsigned long long) * kCodes, s);
+    k_code_reps<<<grid, 256, 0, s>>>(codes, n, dreps);
+    std::vector<unsigned long long> reps(kCodes);
+    std::vector<int64_t> rep(kCodes);
+    if(hipMemcpyAsync(reps.data(), dreps, sizeof(unsigned long long) * kCodes, hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess) {
+        cleanup();
+        mono_set_error("HIP error building the operator's row codes");
+        return -1;
+    }
+    for(int i = 0; i < kCodes; i++) rep[i] = reps[i] == ~0ull ? -1 : (int64_t)reps[i];
+    std::vector<double> table27((size_t)kCodes * 27);
+    unsigned long long mismatches = 0;
+    if(hipMemcpyAsync(drep, rep.data(), sizeof(int64_t) * kCodes, hipMemcpyHostToDevice, s) != hipSuccess) {
+        cleanup();
+        mono_set_error("HIP error building the operator's row codes");
+        return -1;
+    }
+    k_gather_rows<<<(kCodes * 27 + 255) / 256, 256, 0, s>>>(coef27, n, drep, kCodes, dtable27);
+    (void)hipMemsetAsync(dmis, 0, sizeof(unsigned long long), s);
+    k_verify_codes<<<grid, 256, 0, s>>>(coef27, n, codes, dtable27, dmis);
+    if(hipGetLastError() != hipSuccess ||
+       hipMemcpyAsync(table27.data(), dtable27, sizeof(double) * kCodes * 27, hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipMemcpyAsync(&mismatches, dmis, sizeof(mismatches), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess) {
+        cleanup();
+        mono_set_error("HIP error verifying the operator's row codes");
+        return -1;
+    }
+    if(mismatches != 0) {
+        cleanup();
+        return 0;
+    }
+    std::vector<double> t64((size_t)kCodes * 7);
+    std::vector<float> t32((size_t)kCodes * 7);
+    for(int c = 0; c < kCodes; c++)
+        for(int k = 0; k < 7; k++) {
+            t64[(size_t)c * 7 + k] = table27[(size_t)c * 27 + pslots[k]];
+            t32[(size_t)c * 7 + k] = (float)t64[(size_t)c * 7 + k];
+        }
+    (void)hipFree(dreps);
+    (void)hipFree(dmis);
+    (void)hipFree(drep);
+    (void)hipFree(dtable27);
+    op->codes = codes;
+    if(hipMalloc(&op->ctab64, t64.size() * sizeof(double)) != hipSuccess || hipMalloc(&op->ctab32, t32.size() * sizeof(float)) != hipSuccess ||
+       hipMemcpyAsync(op->ctab64, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(op->ctab32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess) {
+        mono_set_error("HIP error uploading the operator's code table");
+        return -1;
+    }
+    return 1;
+}
+
 template <typename T>
 static OpView<T> make_view(const Operator &o, const T *table, const T *tint, const T *percell) {
     OpView<T> v;
@@ -409,18 +529,34 @@ static OpView<T> make_view(const Operator &o, const T *table, const T *tint, con
 }
 
 OpView<double> Operator::view64() const {
+    if(code_mode) {
+        OpView<double> v = make_view<double>(*this, nullptr, nullptr, nullptr);
+        v.codes = codes;
+        v.ctab = ctab64;
+        return v;
+    }
     return class_mode ? make_view<double>(*this, table64, tint64, nullptr) : make_view<double>(*this, nullptr, nullptr, percell64);
 }
 OpView<float> Operator::view32() const {
+    if(code_mode) {
+        OpView<float> v = make_view<float>(*this, nullptr, nullptr, nullptr);
+        v.codes = codes;
+        v.ctab = ctab32;
+        return v;
+    }
     return class_mode ? make_view<float>(*this, table32, tint32, nullptr) : make_view<float>(*this, nullptr, nullptr, percell32);
 }
 
 int Operator::release() {
-    void *bufs[] = {table64, tint64, table32, tint32, percell64, percell32};
+    void *bufs[] = {table64, tint64, table32, tint32, percell64, percell32, codes, ctab64, ctab32};
     for(void *p : bufs)
         if(p) MONO_CHECK(hipFree(p));
     table64 = tint64 = percell64 = nullptr;
     table32 = tint32 = percell32 = nullptr;
+    codes = nullptr;
+    ctab64 = nullptr;
+    ctab32 = nullptr;
+    code_mode = false;
     return 0;
 }
 
@@ -555,6 +691,14 @@ int build_operator(const AssemblyParams &p, const double *coef27, Operator *op, 
         op->nint = 0;
         std::vector<int> pslots = bslots;
         op->pat = pb;
+        if(pb == 7) {
+            const int rc = try_code_mode(p, coef27, pslots, op, s);
+            if(rc < 0) return -1;
+            if(rc == 1) {
+                op->code_mode = true;
+                return 0;
+            }
+        }
         int *dslots = nullptr;
         MONO_CHECK(hipMalloc(&dslots, sizeof(int) * op->pat));
         MONO_CHECK(hipMemcpyAsync(dslots, pslots.data(), sizeof(int) * op->pat, hipMemcpyHostToDevice, s));
