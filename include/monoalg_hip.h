@@ -28,6 +28,8 @@
  *                           caller-given states and per-cell stimuli
  *                           (ode_solver.h:37-47; the GPU plugins'
  *                           solve_model_odes_gpu take a host stim array).
+ *   mono_ionic_run_adaptive the same with the adaptive Euler solver
+ *                           (solve_forward_euler_cpu_adpt, default_solvers.c:61-176).
  *   mono_assemble           ASSEMBLY_MATRIX rows (matrix_assembly.c:334-514) as a
  *                           dense 27-point stencil per cell.
  *   mono_cg_stencil /       SOLVE_LINEAR_SYSTEM conjugate_gradient
@@ -55,7 +57,8 @@ extern "C" {
 enum {
     MONO_MODEL_TT2006 = 0,  /* libten_tusscher_2006.so, 19 states */
     MONO_MODEL_ORD2011 = 1, /* libohara_rudy_endo_2011.so, 41 states */
-    MONO_MODEL_MS2003 = 2   /* libmitchell_shaeffer_2003.so, 2 states */
+    MONO_MODEL_MS2003 = 2,  /* libmitchell_shaeffer_2003.so, 2 states */
+    MONO_MODEL_CRN1998 = 3  /* libcourtemanche_ramirez_nattel_1998.so, 21 states, forward or adaptive Euler */
 };
 
 /* arithmetic of the ionic step and of the diffusion solve */
@@ -103,6 +106,11 @@ typedef struct mono_tissue_desc {
      * GCaL, GKs (4th unused); MS2003 tau_close (2nd-4th unused); all 1 is the
      * reference model.  No counterpart in the reference (batch sweeps). */
     const float *ionic_scale;
+    /* [ode_solver] adaptive=true (ode_solver.c:308-328; CRN 1998 only): the
+     * adaptive Euler solver with min dt = dt_ode and max dt = dt_pde; each step
+     * advances every cell to t + dt_ode with its own step size */
+    int ode_adaptive;
+    double ode_abs_tol, ode_rel_tol;  /* [ode_solver] abs_tol / rel_tol (reference defaults 1e-5) */
 } mono_tissue_desc;
 
 typedef struct mono_stats {
@@ -165,6 +173,14 @@ int mono_tissue_get_activations(mono_tissue *t, int32_t *n_act, float *act, int3
  * stimulus (n doubles, may be NULL), nsteps ODE steps of dt.  precision selects
  * the FP64 or FP32 kernel. */
 int mono_ionic_run(int model, int precision, double *states, int64_t n, const double *stim, double dt, int nsteps);
+/* Standalone adaptive-Euler solve (solve_model_odes_cpu with ode_solver->adaptive,
+ * default_solvers.c:9-46,61-176): advances n cells (states AoS n x neq, in/out)
+ * to current_t + min_dt, each from its own time; ode_dt, ode_time_new and
+ * ode_previous_dt are the reference's per-cell arrays (n doubles each, in/out;
+ * start with dt = min_dt, time_new = previous_dt = 0).  CRN 1998 only. */
+int mono_ionic_run_adaptive(int model, int precision, double *states, int64_t n, const double *stim, double current_t,
+                            double *ode_dt, double *ode_time_new, double *ode_previous_dt, double min_dt, double max_dt,
+                            double abs_tol, double rel_tol);
 int mono_model_neq(int model);
 /* default initial state (neq doubles) and INITIAL_V of a model */
 int mono_model_initial_state(int model, double *out);

@@ -15,10 +15,11 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (FP32_REFINE, FP64, MODEL_MS2003, MODEL_ORD2011, MODEL_TT2006, TENSOR_PER_CELL,  # noqa: F401
+from ._lib import (FP32_REFINE, FP64, MODEL_CRN1998, MODEL_MS2003, MODEL_ORD2011, MODEL_TT2006,  # noqa: F401
+                   TENSOR_PER_CELL,
                    TENSOR_PER_LAYER, TENSOR_UNIFORM, MonoError, Stats, TissueDesc, check, lib, ptr)
 
-MODELS = {"tt2006": MODEL_TT2006, "ord2011": MODEL_ORD2011, "ms2003": MODEL_MS2003}
+MODELS = {"tt2006": MODEL_TT2006, "ord2011": MODEL_ORD2011, "ms2003": MODEL_MS2003, "crn1998": MODEL_CRN1998}
 
 
 def model_neq(model: int) -> int:
@@ -76,11 +77,13 @@ def layered_fibre_tensors(nz: int, theta0_deg: float, theta1_deg: float, sigma_l
 def make_desc(shape, h=(100.0, 100.0, 100.0), model=MODEL_TT2006, precision=FP64, dt=0.02, dt_ode=None, beta=0.14,
               cm=1.0, aniso=False, tensor=None, tensor_source=TENSOR_UNIFORM, g_cell=None, active=None,
               cg_tol=1e-16, cg_rel_tol=0.0, cg_max_its=200, jacobi=False, inner_rel=1e-4,
-              stim=(0.0, 2.0, -50.0, 500.0, 0.0), stim_mask=None, ionic_scale=None):
+              stim=(0.0, 2.0, -50.0, 500.0, 0.0), stim_mask=None, ionic_scale=None, adaptive=None):
     """Build a TissueDesc; returns (desc, keepalive) where keepalive holds the arrays.
 
     stim_mask (n,) 0/1 replaces the x < stim_x_limit rule; ionic_scale (n, 4)
-    float32 scales TT2006 GKr/GCaL/GKs or MS2003 tau_close per cell."""
+    float32 scales TT2006 GKr/GCaL/GKs or MS2003 tau_close per cell.
+    adaptive: None, or (abs_tol, rel_tol) for the adaptive Euler ODE solver
+    (min dt = dt_ode, max dt = dt; CRN 1998)."""
     d = TissueDesc()
     d.nx, d.ny, d.nz = shape
     d.dx, d.dy, d.dz = h
@@ -115,6 +118,9 @@ def make_desc(shape, h=(100.0, 100.0, 100.0), model=MODEL_TT2006, precision=FP64
             raise ValueError(f"ionic_scale has {ionic_scale.size} entries for 4 x {n} cells")
         keep.append(ionic_scale)
         d.ionic_scale = ionic_scale.ctypes.data
+    if adaptive is not None:
+        d.ode_adaptive = 1
+        d.ode_abs_tol, d.ode_rel_tol = adaptive
     d.cg_tol, d.cg_rel_tol, d.cg_max_its, d.cg_jacobi = cg_tol, cg_rel_tol, cg_max_its, int(jacobi)
     d.inner_rel = inner_rel
     d.stim_start, d.stim_duration, d.stim_current, d.stim_x_limit, d.stim_period = stim
@@ -241,6 +247,20 @@ def ionic_run(model: int, states: np.ndarray, stim: np.ndarray | None, dt: float
     st = None if stim is None else np.ascontiguousarray(stim, np.float64)
     check(lib().mono_ionic_run(model, precision, ptr(s), n, ptr(st), dt, nsteps), "mono_ionic_run")
     return s
+
+
+def ionic_run_adaptive(model: int, states: np.ndarray, aux: np.ndarray, stim, current_t: float, min_dt: float,
+                       max_dt: float, abs_tol: float = 1e-5, rel_tol: float = 1e-5, precision: int = FP64):
+    """One adaptive-Euler call (to current_t + min_dt) on AoS states (n, neq) and
+    per-cell aux (n, 3) = (dt, time_new, previous_dt); returns new (states, aux)."""
+    s = np.array(states, np.float64, copy=True, order="C")
+    n = s.shape[0]
+    a = np.array(aux, np.float64, copy=True).reshape(n, 3)
+    dt, tn, pdt = (np.ascontiguousarray(a[:, k]) for k in range(3))
+    st = None if stim is None else np.ascontiguousarray(stim, np.float64)
+    check(lib().mono_ionic_run_adaptive(model, precision, ptr(s), n, ptr(st), current_t, ptr(dt), ptr(tn), ptr(pdt), min_dt,
+                                        max_dt, abs_tol, rel_tol), "mono_ionic_run_adaptive")
+    return s, np.stack([dt, tn, pdt], axis=1)
 
 
 def assemble(shape, **kw):

@@ -13,7 +13,7 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libmonoalg_hip.so")
 
-MODEL_TT2006, MODEL_ORD2011, MODEL_MS2003 = 0, 1, 2
+MODEL_TT2006, MODEL_ORD2011, MODEL_MS2003, MODEL_CRN1998 = 0, 1, 2, 3
 FP64, FP32_REFINE = 0, 1
 TENSOR_UNIFORM, TENSOR_PER_LAYER, TENSOR_PER_CELL = 0, 1, 2
 
@@ -24,7 +24,7 @@ EXPORTED = (
     "mono_tissue_get_state", "mono_tissue_set_state", "mono_tissue_get_stats", "mono_tissue_set_timing",
     "mono_tissue_device_ptrs", "mono_ionic_run", "mono_model_neq", "mono_model_initial_state",
     "mono_model_initial_v", "mono_assemble", "mono_cg_stencil", "mono_cg_csr",
-    "mono_tissue_observe_setup", "mono_tissue_observe", "mono_tissue_get_activations",
+    "mono_tissue_observe_setup", "mono_tissue_observe", "mono_tissue_get_activations", "mono_ionic_run_adaptive",
 )
 
 
@@ -46,6 +46,7 @@ class TissueDesc(C.Structure):
         ("stim_start", C.c_double), ("stim_duration", C.c_double), ("stim_current", C.c_double),
         ("stim_x_limit", C.c_double), ("stim_period", C.c_double),
         ("stim_mask", C.c_void_p), ("ionic_scale", C.c_void_p),
+        ("ode_adaptive", C.c_int), ("ode_abs_tol", C.c_double), ("ode_rel_tol", C.c_double),
     ]
 
 
@@ -92,6 +93,7 @@ def lib() -> C.CDLL:
     L.mono_tissue_set_timing.argtypes = [vp, C.c_int]
     L.mono_tissue_device_ptrs.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_size_t), C.POINTER(vp)]
     L.mono_ionic_run.argtypes = [C.c_int, C.c_int, vp, i64, vp, C.c_double, C.c_int]
+    L.mono_ionic_run_adaptive.argtypes = [C.c_int, C.c_int, vp, i64, vp, C.c_double, vp, vp, vp] + [C.c_double] * 4
     L.mono_model_neq.argtypes = [C.c_int]
     L.mono_model_initial_state.argtypes = [C.c_int, vp]
     L.mono_model_initial_v.argtypes = [C.c_int]

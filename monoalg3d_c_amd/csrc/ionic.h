@@ -25,10 +25,23 @@ template <typename T> struct IonicLaunch {
     int num_steps;          // ODE steps per PDE step
     int first_step;         // read V from sv row 0 instead of v (t = 0: no Vm mirror)
     int write_vm_row;       // also store V* in sv row 0
+    // adaptive Euler (launch_ionic_adaptive): per-cell dt, time_new and
+    // previous_dt in three FP64 rows of `pitch`; each cell advances from its
+    // time_new to final_time
+    double *aux;
+    double final_time, min_dt, max_dt, abs_tol, rel_tol;
 };
+
+// Substep cap of one adaptive call per cell: a cell that would need more (a
+// non-finite state, whose time never reaches final_time, or a step size held
+// at a tiny min_dt for a whole call) stops with V = NaN, which the tissue's
+// NaN check reports, instead of looping on.
+constexpr int kMaxAdaptiveSubsteps = 1 << 20;
 
 hipError_t launch_ionic_step(int model, const IonicLaunch<double> &a, hipStream_t s);
 hipError_t launch_ionic_step(int model, const IonicLaunch<float> &a, hipStream_t s);
+hipError_t launch_ionic_adaptive(int model, const IonicLaunch<double> &a, hipStream_t s);
+hipError_t launch_ionic_adaptive(int model, const IonicLaunch<float> &a, hipStream_t s);
 hipError_t launch_ionic_init(int model, double *sv, size_t pitch, double *v, int64_t n, const double *ic, hipStream_t s);
 hipError_t launch_ionic_init(int model, float *sv, size_t pitch, double *v, int64_t n, const float *ic, hipStream_t s);
 void default_initial_state(int model, double *out);

@@ -14,6 +14,7 @@
 // Row 0 (V) is only written when `write_vm_row` is set (the reference's plugin
 // contract that row 0 holds Vm after SOLVE_MODEL_ODES).
 #include "ionic.h"
+#include "ionic_crn1998.h"
 #include "ionic_models.h"
 #include "mono_common.h"
 
@@ -23,6 +24,13 @@ template <int MODEL> struct ModelTraits;
 template <> struct ModelTraits<kTT2006> { static constexpr int neq = 19; };
 template <> struct ModelTraits<kORd2011> { static constexpr int neq = 41; };
 template <> struct ModelTraits<kMS2003> { static constexpr int neq = 2; };
+template <> struct ModelTraits<kCRN1998> { static constexpr int neq = kCRNNeq; };
+
+// dy = f(y) for the models integrated by (adaptive) forward Euler
+template <int MODEL, typename T> __device__ __forceinline__ void model_rhs(const T *y, T *dy, T stim) {
+    static_assert(MODEL == kCRN1998, "adaptive Euler runs the models given by their full RHS");
+    crn1998_rhs<T>(y, dy, stim);
+}
 
 template <int MODEL, typename T>
 __device__ __forceinline__ T model_step(T *y, T stim, T dt, const float *scale) {
@@ -30,9 +38,29 @@ __device__ __forceinline__ T model_step(T *y, T stim, T dt, const float *scale) 
         return tt2006_step<T>(y, stim, dt, T(scale[0]), T(scale[1]), T(scale[2]));
     } else if constexpr(MODEL == kORd2011) {
         return ord2011_step<T>(y, stim, dt);
+    } else if constexpr(MODEL == kCRN1998) {
+        return crn1998_step<T>(y, stim, dt);
     } else {
         return ms2003_step<T>(y, stim, dt, T(scale[0]));
     }
+}
+
+// stimulus for cell i: per-cell array, else (inside the stimulus window) the
+// per-cell mask or the x-limit rule on the cell centre
+template <typename T> __device__ __forceinline__ double cell_stim(const IonicLaunch<T> &a, int64_t i) {
+    double st = 0.0;
+    if(a.stim_cell) {
+        st = a.stim_cell[i];
+    } else if(a.stim_amp != 0.0) {
+        if(a.stim_mask) {
+            if(a.stim_mask[i]) st = a.stim_amp;
+        } else {
+            const int ix = (int)(i % a.nx);
+            const double cx = ix * a.dx + a.dx / 2.0;
+            if(cx < a.stim_x_limit) st = a.stim_amp;
+        }
+    }
+    return st;
 }
 
 template <int MODEL, typename T>
@@ -47,20 +75,7 @@ __device__ __forceinline__ void ionic_step_body(const IonicLaunch<T> &a) {
     for(int s = 1; s < NEQ; s++) y[s] = a.sv[(size_t)s * a.pitch + i];
     double V = a.first_step ? (double)a.sv[i] : a.v[i];
 
-    // stimulus for this cell: per-cell array, else (inside the stimulus window)
-    // the per-cell mask or the x-limit rule on the cell centre
-    double st = 0.0;
-    if(a.stim_cell) {
-        st = a.stim_cell[i];
-    } else if(a.stim_amp != 0.0) {
-        if(a.stim_mask) {
-            if(a.stim_mask[i]) st = a.stim_amp;
-        } else {
-            const int ix = (int)(i % a.nx);
-            const double cx = ix * a.dx + a.dx / 2.0;
-            if(cx < a.stim_x_limit) st = a.stim_amp;
-        }
-    }
+    const double st = cell_stim(a, i);
     float scale[4] = {1.f, 1.f, 1.f, 1.f};
     if(a.scale) {
 #pragma unroll
@@ -76,6 +91,96 @@ __device__ __forceinline__ void ionic_step_body(const IonicLaunch<T> &a) {
     if(a.write_vm_row) a.sv[i] = T(V);
     // fused UPDATE_MONODOMAIN: b = Vm * alpha
     if(a.b) a.b[i] = V * a.alpha;
+}
+
+// Adaptive forward Euler (solve_forward_euler_cpu_adpt, default_solvers.c:61-176)
+// for models given by their full RHS.  Each lane advances its cell from its own
+// time_new to final_time: Euler steps checked by a second RHS evaluation at the
+// new point, error = max_i |dt/2 (k1_i - k2_i)| / max(abs_tol, rel_tol |y_new_i|)
+// (+ abs_tol^2), next dt = 0.8 dt / sqrt(error) clamped to [min_dt, max_dt],
+// a step rejected while error >= 1 and dt > min_dt, and the reference's time
+// bookkeeping (including its t + dt - dt round trip around the second RHS),
+// so that the sequence of step sizes is the reference's.  V is carried in
+// FP64 (it is the PDE unknown); the other states in T.
+template <int MODEL, typename T>
+__device__ __forceinline__ void adaptive_body(const IonicLaunch<T> &a) {
+    constexpr int NEQ = ModelTraits<MODEL>::neq;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= a.n) return;
+    if(a.active && !a.active[i]) return;
+    T y[NEQ], k1[NEQ], k2[NEQ], yp[NEQ];
+#pragma unroll
+    for(int s = 1; s < NEQ; s++) y[s] = a.sv[(size_t)s * a.pitch + i];
+    double V = a.first_step ? (double)a.sv[i] : a.v[i];
+    const T st = T(cell_stim(a, i));
+    double *arow = a.aux;
+    double dt = arow[i], tnew = arow[a.pitch + i], pdt;
+    const double final_time = a.final_time, min_dt = a.min_dt, max_dt = a.max_dt;
+    const double abs_tol = a.abs_tol, rel_tol = a.rel_tol, tiny = pow(abs_tol, 2.0);
+    pdt = dt;
+    if(tnew + dt > final_time) dt = final_time - tnew;
+    bool ok = false;
+    y[0] = T(V);
+    model_rhs<MODEL, T>(y, k1, st);
+    tnew += dt;
+    for(int it = 0; it < kMaxAdaptiveSubsteps; it++) {
+        // Euler step from the accepted state and the RHS at its end
+        const double Vp = (double)k1[0] * dt + V;
+#pragma unroll
+        for(int s = 1; s < NEQ; s++) yp[s] = k1[s] * T(dt) + y[s];
+        yp[0] = T(Vp);
+        tnew += dt;
+        model_rhs<MODEL, T>(yp, k2, st);
+        tnew -= dt;
+        double err = 0.0;
+#pragma unroll
+        for(int s = 0; s < NEQ; s++) {
+            const double yn = s == 0 ? Vp : (double)yp[s];
+            const double at = fabs(yn) * rel_tol;
+            const double tol = (abs_tol > at) ? abs_tol : at;
+            const double e = fabs(((dt / 2.0) * ((double)k1[s] - (double)k2[s])) / tol);
+            err = (e > err) ? e : err;
+        }
+        err += tiny;
+        pdt = dt;
+        dt = 0.8 * dt * sqrt(1.0 / err);
+        if(dt < min_dt) dt = min_dt;
+        else if(dt > max_dt) dt = max_dt;
+        if(tnew + dt > final_time) dt = final_time - tnew;
+        if(err >= 1.0 && dt > min_dt) continue;  // rejected: y, V and k1 stay
+        V = Vp;
+#pragma unroll
+        for(int s = 0; s < NEQ; s++) {
+            k1[s] = k2[s];
+            if(s > 0) y[s] = yp[s];
+        }
+        if(tnew + pdt >= final_time) {
+            if(final_time == tnew) {
+                ok = true;
+                break;
+            } else if(tnew < final_time) {
+                dt = pdt = final_time - tnew;
+                tnew += pdt;
+                ok = true;
+                break;
+            }
+        } else {
+            tnew += pdt;
+        }
+    }
+    if(!ok) V = __longlong_as_double(0x7ff8000000000000ll);  // substep cap reached: report NaN
+    arow[i] = dt;
+    arow[a.pitch + i] = tnew;
+    arow[2 * a.pitch + i] = pdt;
+#pragma unroll
+    for(int s = 1; s < NEQ; s++) a.sv[(size_t)s * a.pitch + i] = y[s];
+    if(a.write_vm_row) a.sv[i] = T(V);
+    if(a.b) a.b[i] = V * a.alpha;
+}
+
+template <int MODEL, typename T>
+__global__ __launch_bounds__(256) void k_ionic_adaptive(IonicLaunch<T> a) {
+    adaptive_body<MODEL, T>(a);
 }
 
 // FP32: the register count the compiler picks.  FP64 models are bound by
@@ -102,7 +207,8 @@ __global__ __launch_bounds__(256) void k_ionic_init(T *sv, size_t pitch, double 
 }
 
 // Default initial conditions (ten_tusscher_2006_RS_CPU.c:28-46, Ohara_Rudy_2011.c:25-65,
-// mitchell_shaeffer_2003.c:24-25); float literals where the reference has them.
+// mitchell_shaeffer_2003.c:24-25, courtemanche_ramirez_nattel_1998.c:31-51);
+// float literals where the reference has them.
 void default_initial_state(int model, double *out) {
     if(model == kTT2006) {
         static const float ic[19] = {-85.23f, 0.00621f, 0.4712f, 0.0095f, 0.00172f, 0.7444f, 0.7045f,
@@ -114,6 +220,13 @@ void default_initial_state(int model, double *out) {
                                       1, 0, 1, 1, 0, 1, 1, 0, 1, 1, 0, 1, 1, 1, 1, 1, 0, 1, 1, 0, 0, 0,
                                       0, 1, 0, 0, 0};
         for(int s = 0; s < 41; s++) out[s] = ic[s];
+    } else if(model == kCRN1998) {
+        static const float ic[21] = {-8.118000e+01f, 2.908000e-03f, 9.649000e-01f, 9.775000e-01f, 3.043000e-02f,
+                                     9.992000e-01f,  4.966000e-03f, 9.986000e-01f, 3.296000e-05f, 1.869000e-02f,
+                                     1.367000e-04f,  9.996000e-01f, 7.755000e-01f, 0.0f,          1.000000e+00f,
+                                     9.992000e-01f,  1.117000e+01f, 1.390000e+02f, 1.013000e-04f, 1.488000e+00f,
+                                     1.488000e+00f};
+        for(int s = 0; s < 21; s++) out[s] = (double)ic[s];
     } else {
         out[0] = (double)0.00000820413566106744f;
         out[1] = (double)0.8789655121804799f;
@@ -125,6 +238,7 @@ void default_initial_state(int model, double *out) {
 double model_initial_v(int model) {
     if(model == kTT2006) return (double)(-85.23f);
     if(model == kORd2011) return (double)(-85.f);
+    if(model == kCRN1998) return (double)(-81.180000f);  // courtemanche_ramirez_nattel_1998.h:7
     return 0.00000820413566106744;
 }
 
@@ -138,6 +252,7 @@ static hipError_t launch_step_t(int model, const IonicLaunch<T> &a, hipStream_t 
         case kTT2006: k_ionic_step64<kTT2006><<<grid, block, 0, s>>>(a); break;
         case kORd2011: k_ionic_step64<kORd2011><<<grid, block, 0, s>>>(a); break;
         case kMS2003: k_ionic_step64<kMS2003><<<grid, block, 0, s>>>(a); break;
+        case kCRN1998: k_ionic_step64<kCRN1998><<<grid, block, 0, s>>>(a); break;
         default: return hipErrorInvalidValue;
         }
     } else {
@@ -145,11 +260,28 @@ static hipError_t launch_step_t(int model, const IonicLaunch<T> &a, hipStream_t 
         case kTT2006: k_ionic_step32<kTT2006><<<grid, block, 0, s>>>(a); break;
         case kORd2011: k_ionic_step32<kORd2011><<<grid, block, 0, s>>>(a); break;
         case kMS2003: k_ionic_step32<kMS2003><<<grid, block, 0, s>>>(a); break;
+        case kCRN1998: k_ionic_step32<kCRN1998><<<grid, block, 0, s>>>(a); break;
         default: return hipErrorInvalidValue;
         }
     }
     return hipGetLastError();
 }
+
+template <typename T>
+static hipError_t launch_adaptive_t(int model, const IonicLaunch<T> &a, hipStream_t s) {
+    const int block = 256;
+    const unsigned grid = (unsigned)((a.n + block - 1) / block);
+    if(a.n == 0) return hipSuccess;
+    if(!a.aux) return hipErrorInvalidValue;
+    switch(model) {
+    case kCRN1998: k_ionic_adaptive<kCRN1998, T><<<grid, block, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ionic_adaptive(int model, const IonicLaunch<double> &a, hipStream_t s) { return launch_adaptive_t(model, a, s); }
+hipError_t launch_ionic_adaptive(int model, const IonicLaunch<float> &a, hipStream_t s) { return launch_adaptive_t(model, a, s); }
 
 hipError_t launch_ionic_step(int model, const IonicLaunch<double> &a, hipStream_t s) { return launch_step_t(model, a, s); }
 hipError_t launch_ionic_step(int model, const IonicLaunch<float> &a, hipStream_t s) { return launch_step_t(model, a, s); }
@@ -163,6 +295,7 @@ static hipError_t launch_init_t(int model, T *sv, size_t pitch, double *v, int64
     case kTT2006: k_ionic_init<kTT2006, T><<<grid, block, 0, s>>>(sv, pitch, v, n, ic_dev); break;
     case kORd2011: k_ionic_init<kORd2011, T><<<grid, block, 0, s>>>(sv, pitch, v, n, ic_dev); break;
     case kMS2003: k_ionic_init<kMS2003, T><<<grid, block, 0, s>>>(sv, pitch, v, n, ic_dev); break;
+    case kCRN1998: k_ionic_init<kCRN1998, T><<<grid, block, 0, s>>>(sv, pitch, v, n, ic_dev); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

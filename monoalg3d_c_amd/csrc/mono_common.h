@@ -22,11 +22,14 @@ namespace mono {
 constexpr int kWave = 64;
 
 // Ionic model identifiers (stable; part of the C ABI).
-enum Model : int { kTT2006 = 0, kORd2011 = 1, kMS2003 = 2 };
+enum Model : int { kTT2006 = 0, kORd2011 = 1, kMS2003 = 2, kCRN1998 = 3 };
+constexpr int kNumModels = 4;
 
 // number of state variables per model (ten_tusscher_2006.h:6, Ohara_Rudy_2011.h:7,
-// mitchell_shaeffer_2003.h:7)
-__host__ __device__ constexpr int model_neq(int m) { return m == kTT2006 ? 19 : (m == kORd2011 ? 41 : 2); }
+// mitchell_shaeffer_2003.h:7, courtemanche_ramirez_nattel_1998.h:6)
+__host__ __device__ constexpr int model_neq(int m) {
+    return m == kTT2006 ? 19 : (m == kORd2011 ? 41 : (m == kMS2003 ? 2 : 21));
+}
 
 // 27-point stencil slot of the neighbour offset (ox, oy, oz); 13 is the cell itself.
 __host__ __device__ constexpr int slot_of(int ox, int oy, int oz) { return (oz + 1) * 9 + (oy + 1) * 3 + (ox + 1); }

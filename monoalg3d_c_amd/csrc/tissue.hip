@@ -44,6 +44,7 @@ struct mono_tissue {
     double *tensor = nullptr;
     uint8_t *stim_mask = nullptr;
     float *ionic_scale = nullptr;
+    double *aux = nullptr;  // adaptive ODE solver: dt, time_new, previous_dt rows of `pitch`
     Operator op;
     CgWorkspace cg;
     CgConfig cgcfg;
@@ -81,9 +82,19 @@ static int validate(const mono_tissue_desc *d) {
         mono_set_error("dx, dy, dz and dt_pde must be positive");
         return -1;
     }
-    if(d->model < 0 || d->model > 2) {
+    if(d->model < 0 || d->model >= kNumModels) {
         mono_set_error("unknown ionic model %d", d->model);
         return -1;
+    }
+    if(d->ode_adaptive) {
+        if(d->model != kCRN1998) {
+            mono_set_error("the adaptive ODE solver runs CRN 1998 (model %d given)", d->model);
+            return -1;
+        }
+        if(!(d->dt_ode > 0) || d->dt_ode > d->dt_pde || d->ode_abs_tol < 0 || d->ode_rel_tol < 0) {
+            mono_set_error("adaptive ODE solver: need 0 < dt_ode <= dt_pde and non-negative tolerances");
+            return -1;
+        }
     }
     if(d->precision != MONO_FP64 && d->precision != MONO_FP32_REFINE) {
         mono_set_error("unknown precision %d", d->precision);
@@ -135,6 +146,17 @@ static int assemble_into(const mono_tissue_desc &d, int64_t n, double **tensor_d
     return 0;
 }
 
+// adaptive call of this step: every cell to cur_time + min dt
+// (solve_model_odes_cpu, default_solvers.c:9-46; max dt = dt_pde, ode_solver.c:311-328)
+template <typename T> static void set_adaptive(const mono_tissue *t, IonicLaunch<T> &a) {
+    a.aux = t->aux;
+    a.final_time = t->cur_time + t->dt_ode;
+    a.min_dt = t->dt_ode;
+    a.max_dt = t->d.dt_pde;
+    a.abs_tol = t->d.ode_abs_tol;
+    a.rel_tol = t->d.ode_rel_tol;
+}
+
 extern "C" {
 
 const char *mono_last_error(void) { return g_err; }
@@ -145,10 +167,10 @@ int mono_device_count(void) {
     return c;
 }
 
-int mono_model_neq(int model) { return (model >= 0 && model <= 2) ? model_neq(model) : -1; }
+int mono_model_neq(int model) { return (model >= 0 && model < kNumModels) ? model_neq(model) : -1; }
 
 int mono_model_initial_state(int model, double *out) {
-    if(model < 0 || model > 2 || !out) {
+    if(model < 0 || model >= kNumModels || !out) {
         mono_set_error("bad model or output");
         return -1;
     }
@@ -156,7 +178,7 @@ int mono_model_initial_state(int model, double *out) {
     return 0;
 }
 
-double mono_model_initial_v(int model) { return model_initial_v(model); }
+double mono_model_initial_v(int model) { return (model >= 0 && model < kNumModels) ? model_initial_v(model) : NAN; }
 
 int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out) {
     if(validate(desc)) return -1;
@@ -177,7 +199,13 @@ int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out) {
     t->pitch = (size_t)((t->n + 63) / 64 * 64);
     t->alpha = alpha_of(d);
     t->dt_ode = d.dt_ode > 0 ? d.dt_ode : d.dt_pde;
-    t->num_ode_steps = d.dt_pde >= t->dt_ode ? (int)(d.dt_pde / t->dt_ode) : 1;
+    // sub-steps per PDE step (monodomain_solver.c:510-520); one adaptive call
+    t->num_ode_steps = d.ode_adaptive ? 1 : (d.dt_pde >= t->dt_ode ? (int)(d.dt_pde / t->dt_ode) : 1);
+    if(d.ode_adaptive) {
+        // reference defaults of [ode_solver] abs_tol / rel_tol (config_parser.c:307-311)
+        if(t->d.ode_abs_tol == 0) t->d.ode_abs_tol = 1e-5;
+        if(t->d.ode_rel_tol == 0) t->d.ode_rel_tol = 1e-5;
+    }
     t->stim_start = d.stim_start;
     t->cgcfg.tol = d.cg_tol;
     t->cgcfg.rel_tol = d.cg_rel_tol;
@@ -207,6 +235,18 @@ int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out) {
         mono_tissue_destroy(t);
         return -1;
     };
+
+    // adaptive ODE solver rows: dt = min dt, time_new = previous_dt = 0 (as the
+    // models' set_model_initial_conditions_cpu, courtemanche_ramirez_nattel_1998.c:12-24)
+    if(d.ode_adaptive) {
+        std::vector<double> haux(3 * t->pitch, 0.0);
+        for(int64_t i = 0; i < t->n; i++) haux[i] = t->dt_ode;
+        if(hipMalloc(&t->aux, sizeof(double) * 3 * t->pitch) != hipSuccess ||
+           hipMemcpy(t->aux, haux.data(), sizeof(double) * 3 * t->pitch, hipMemcpyHostToDevice) != hipSuccess) {
+            mono_set_error("adaptive solver state allocation failed");
+            return fail();
+        }
+    }
 
     // per-cell stimulus mask and model scalings
     if(d.stim_mask) {
@@ -291,7 +331,7 @@ int mono_tissue_destroy(mono_tissue *t) {
     if(t->cg.release()) rc = -1;
     if(t->op.release()) rc = -1;
     if(t->obs.release()) rc = -1;
-    void *bufs[] = {t->sv, t->v, t->b, t->active, t->g, t->tensor, t->stim_mask, t->ionic_scale};
+    void *bufs[] = {t->sv, t->v, t->b, t->active, t->g, t->tensor, t->stim_mask, t->ionic_scale, t->aux};
     for(void *p : bufs)
         if(p && hipFree(p) != hipSuccess) rc = -1;
     for(int k = 0; k < 3; k++)
@@ -361,7 +401,12 @@ static int launch_ionic(mono_tissue *t, double stim_amp) {
         a.dt = t->dt_ode;
         a.num_steps = t->num_ode_steps;
         a.first_step = t->st.steps == 0;
-        MONO_CHECK(launch_ionic_step(d.model, a, t->stream));
+        if(d.ode_adaptive) {
+            set_adaptive(t, a);
+            MONO_CHECK(launch_ionic_adaptive(d.model, a, t->stream));
+        } else {
+            MONO_CHECK(launch_ionic_step(d.model, a, t->stream));
+        }
     } else {
         IonicLaunch<float> a{};
         a.sv = (float *)t->sv;
@@ -380,7 +425,12 @@ static int launch_ionic(mono_tissue *t, double stim_amp) {
         a.dt = (float)t->dt_ode;
         a.num_steps = t->num_ode_steps;
         a.first_step = t->st.steps == 0;
-        MONO_CHECK(launch_ionic_step(d.model, a, t->stream));
+        if(d.ode_adaptive) {
+            set_adaptive(t, a);
+            MONO_CHECK(launch_ionic_adaptive(d.model, a, t->stream));
+        } else {
+            MONO_CHECK(launch_ionic_step(d.model, a, t->stream));
+        }
     }
     return 0;
 }
@@ -505,7 +555,7 @@ int mono_tissue_device_ptrs(mono_tissue *t, void **v, void **sv, size_t *pitch, 
 // ---------------------------------------------------------------------------
 
 int mono_ionic_run(int model, int precision, double *states, int64_t n, const double *stim, double dt, int nsteps) {
-    if(model < 0 || model > 2 || !states || n < 0 || nsteps < 0) {
+    if(model < 0 || model >= kNumModels || !states || n < 0 || nsteps < 0) {
         mono_set_error("mono_ionic_run: bad arguments");
         return -1;
     }
@@ -589,6 +639,123 @@ int mono_ionic_run(int model, int precision, double *states, int64_t n, const do
     MONO_CHECK(hipFree(dsv));
     MONO_CHECK(hipFree(dv));
     if(dstim) MONO_CHECK(hipFree(dstim));
+    return 0;
+}
+
+int mono_ionic_run_adaptive(int model, int precision, double *states, int64_t n, const double *stim, double current_t,
+                            double *ode_dt, double *ode_time_new, double *ode_previous_dt, double min_dt, double max_dt,
+                            double abs_tol, double rel_tol) {
+    if(model != kCRN1998) {
+        mono_set_error("mono_ionic_run_adaptive: the adaptive solver runs CRN 1998 (model %d given)", model);
+        return -1;
+    }
+    if(!states || !ode_dt || !ode_time_new || !ode_previous_dt || n < 0 || !(min_dt > 0.0) || !(max_dt >= min_dt) ||
+       !(abs_tol > 0.0) || !(rel_tol >= 0.0)) {
+        mono_set_error("mono_ionic_run_adaptive: bad arguments");
+        return -1;
+    }
+    if(precision != MONO_FP64 && precision != MONO_FP32_REFINE) {
+        mono_set_error("mono_ionic_run_adaptive: unknown precision %d", precision);
+        return -1;
+    }
+    if(mono_device_count() < 1) {
+        mono_set_error("no HIP device available");
+        return -2;
+    }
+    if(n == 0) return 0;
+    const int neq = model_neq(model);
+    const size_t pitch = (size_t)((n + 63) / 64 * 64);
+    const size_t w = precision == MONO_FP64 ? 8 : 4;
+    std::vector<char> h(w * pitch * neq, 0);
+    for(int s = 0; s < neq; s++)
+        for(int64_t i = 0; i < n; i++) {
+            const double val = states[i * neq + s];
+            if(w == 8) ((double *)h.data())[s * pitch + i] = val;
+            else ((float *)h.data())[s * pitch + i] = (float)val;
+        }
+    std::vector<double> haux(3 * pitch, 0.0), hv(n);
+    for(int64_t i = 0; i < n; i++) {
+        haux[i] = ode_dt[i];
+        haux[pitch + i] = ode_time_new[i];
+        haux[2 * pitch + i] = ode_previous_dt[i];
+        hv[i] = states[i * neq];
+    }
+    void *dsv = nullptr;
+    double *dv = nullptr, *dstim = nullptr, *daux = nullptr;
+    auto cleanup = [&]() {
+        void *bufs[] = {dsv, dv, dstim, daux};
+        for(void *p : bufs)
+            if(p) (void)hipFree(p);
+    };
+    hipError_t e = hipMalloc(&dsv, h.size());
+    if(e == hipSuccess) e = hipMalloc(&dv, sizeof(double) * n);
+    if(e == hipSuccess) e = hipMalloc(&daux, sizeof(double) * 3 * pitch);
+    if(e == hipSuccess && stim) e = hipMalloc(&dstim, sizeof(double) * n);
+    if(e == hipSuccess) e = hipMemcpy(dsv, h.data(), h.size(), hipMemcpyHostToDevice);
+    if(e == hipSuccess) e = hipMemcpy(dv, hv.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+    if(e == hipSuccess) e = hipMemcpy(daux, haux.data(), sizeof(double) * 3 * pitch, hipMemcpyHostToDevice);
+    if(e == hipSuccess && stim) e = hipMemcpy(dstim, stim, sizeof(double) * n, hipMemcpyHostToDevice);
+    if(e == hipSuccess) {
+        // final time of the call: current_t + min_dt (default_solvers.c:9-46)
+        const double final_time = current_t + min_dt;
+        if(precision == MONO_FP64) {
+            IonicLaunch<double> a{};
+            a.sv = (double *)dsv;
+            a.pitch = pitch;
+            a.v = dv;
+            a.b = dv;
+            a.alpha = 1.0;
+            a.n = n;
+            a.stim_cell = dstim;
+            a.nx = 1;
+            a.dx = 1.0;
+            a.write_vm_row = 1;
+            a.aux = daux;
+            a.final_time = final_time;
+            a.min_dt = min_dt;
+            a.max_dt = max_dt;
+            a.abs_tol = abs_tol;
+            a.rel_tol = rel_tol;
+            e = launch_ionic_adaptive(model, a, nullptr);
+        } else {
+            IonicLaunch<float> a{};
+            a.sv = (float *)dsv;
+            a.pitch = pitch;
+            a.v = dv;
+            a.b = dv;
+            a.alpha = 1.0;
+            a.n = n;
+            a.stim_cell = dstim;
+            a.nx = 1;
+            a.dx = 1.0;
+            a.write_vm_row = 1;
+            a.aux = daux;
+            a.final_time = final_time;
+            a.min_dt = min_dt;
+            a.max_dt = max_dt;
+            a.abs_tol = abs_tol;
+            a.rel_tol = rel_tol;
+            e = launch_ionic_adaptive(model, a, nullptr);
+        }
+    }
+    if(e == hipSuccess) e = hipDeviceSynchronize();
+    if(e == hipSuccess) e = hipMemcpy(h.data(), dsv, h.size(), hipMemcpyDeviceToHost);
+    if(e == hipSuccess) e = hipMemcpy(hv.data(), dv, sizeof(double) * n, hipMemcpyDeviceToHost);
+    if(e == hipSuccess) e = hipMemcpy(haux.data(), daux, sizeof(double) * 3 * pitch, hipMemcpyDeviceToHost);
+    cleanup();
+    if(e != hipSuccess) {
+        mono_set_error("mono_ionic_run_adaptive: %s", hipGetErrorString(e));
+        return -1;
+    }
+    for(int s = 1; s < neq; s++)
+        for(int64_t i = 0; i < n; i++)
+            states[i * neq + s] = w == 8 ? ((double *)h.data())[s * pitch + i] : (double)((float *)h.data())[s * pitch + i];
+    for(int64_t i = 0; i < n; i++) {
+        states[i * neq] = hv[i];
+        ode_dt[i] = haux[i];
+        ode_time_new[i] = haux[pitch + i];
+        ode_previous_dt[i] = haux[2 * pitch + i];
+    }
     return 0;
 }
 

@@ -1,10 +1,13 @@
 /*
  * Ionic-model plugin for the reference's ODE-solver ABI, backed by
- * libmonoalg_hip.so.  Built three times (MONO_PLUGIN_MODEL = 0, 1, 2) into
- * libmono_hip_tt2006.so, libmono_hip_ord2011.so and libmono_hip_ms2003.so; the
- * reference selects one with
+ * libmonoalg_hip.so.  Built four times (MONO_PLUGIN_MODEL = 0, 1, 2, 3) into
+ * libmono_hip_tt2006.so, libmono_hip_ord2011.so, libmono_hip_ms2003.so and
+ * libmono_hip_crn1998.so; the reference selects one with
  *     [ode_solver]  library_file=<path>/libmono_hip_<model>.so  use_gpu=no
- * and dlsym's the three symbols below (ode_solver.c:179-233).
+ * and dlsym's the three symbols below (ode_solver.c:179-233).  CRN 1998 also
+ * takes [ode_solver] adaptive=true: the per-cell step-size arrays are
+ * allocated here as the reference's default-solver models do
+ * (courtemanche_ramirez_nattel_1998.c:12-24) and passed to the adaptive kernel.
  *
  * Compiled against the reference's own headers (struct ode_solver and the
  * GET_CELL_MODEL_DATA / SET_ODE_INITIAL_CONDITIONS_CPU / SOLVE_MODEL_ODES
@@ -21,7 +24,7 @@
 #include "monoalg_hip.h"
 
 #ifndef MONO_PLUGIN_MODEL
-#error "define MONO_PLUGIN_MODEL (0 tt2006, 1 ord2011, 2 ms2003)"
+#error "define MONO_PLUGIN_MODEL (0 tt2006, 1 ord2011, 2 ms2003, 3 crn1998)"
 #endif
 
 static void fail(const char *what) {
@@ -50,16 +53,38 @@ SET_ODE_INITIAL_CONDITIONS_CPU(set_model_initial_conditions_cpu) {
     }
     for(uint32_t i = 0; i < n; i++)
         for(int s = 0; s < neq; s++) solver->sv[(size_t)i * neq + s] = ic[s];
+    if(solver->adaptive) {
+        if(MONO_PLUGIN_MODEL != MONO_MODEL_CRN1998) {
+            fprintf(stderr, "libmono_hip: adaptive=true needs a default-solver model (libmono_hip_crn1998.so)\n");
+            exit(EXIT_FAILURE);
+        }
+        solver->ode_dt = (real *)malloc(sizeof(real) * n);
+        solver->ode_previous_dt = (real *)calloc(n, sizeof(real));
+        solver->ode_time_new = (real *)calloc(n, sizeof(real));
+        if(!solver->ode_dt || !solver->ode_previous_dt || !solver->ode_time_new) {
+            fprintf(stderr, "libmono_hip: out of host memory\n");
+            exit(EXIT_FAILURE);
+        }
+        for(uint32_t i = 0; i < n; i++) solver->ode_dt[i] = solver->min_dt;
+    }
 }
 
 /* SOLVE_MODEL_ODES (ode_solver.h:37): num_steps steps of min_dt for every cell,
  * stim_currents in cell order (cells_to_solve is only set by adaptive runs) */
 SOLVE_MODEL_ODES(solve_model_odes_cpu) {
     (void)ode_extra_config;
-    (void)current_t;
     if(ode_solver->cells_to_solve) {
         fprintf(stderr, "libmono_hip: adaptive cell lists are not supported\n");
         exit(EXIT_FAILURE);
+    }
+    if(ode_solver->adaptive) {
+        /* one adaptive call to current_t + min_dt (default_solvers.c:9-46) */
+        if(mono_ionic_run_adaptive(MONO_PLUGIN_MODEL, MONO_FP64, ode_solver->sv, (int64_t)ode_solver->num_cells_to_solve,
+                                   stim_currents, current_t, ode_solver->ode_dt, ode_solver->ode_time_new,
+                                   ode_solver->ode_previous_dt, ode_solver->min_dt, ode_solver->max_dt, ode_solver->abs_tol,
+                                   ode_solver->rel_tol))
+            fail("mono_ionic_run_adaptive");
+        return;
     }
     if(mono_ionic_run(MONO_PLUGIN_MODEL, MONO_FP64, ode_solver->sv, (int64_t)ode_solver->num_cells_to_solve, stim_currents,
                       ode_solver->min_dt, (int)ode_solver->num_steps))

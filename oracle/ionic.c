@@ -18,6 +18,12 @@ void orc_tt2006_init_cell(double *y);
 void orc_tt2006_step_scaled(double *y, double stim, double dt, double sKr, double sCaL, double sKs);
 void orc_ord2011_init_cell(double *y);
 void orc_ord2011_step(double *y, double stim, double dt);
+void orc_crn1998_init_cell(double *y);
+void orc_crn1998_step(double *y, double stim, double dt);
+void orc_crn1998_rhs(const double *y, double *dy, double stim);
+typedef void (*orc_rhs_fn)(const double *y, double *dy, double stim);
+int orc_adaptive_euler(orc_rhs_fn rhs, int neq, double *y, double stim, double final_time, double *aux, double min_dt,
+                       double max_dt, double abs_tol, double rel_tol);
 
 static void ms_init_cell(double *y) {
     y[0] = (double)0.00000820413566106744f;
@@ -42,6 +48,7 @@ int orc_model_neq(int model) {
     case ORC_TT2006: return 19;
     case ORC_ORD2011: return 41;
     case ORC_MS2003: return 2;
+    case ORC_CRN1998: return 21;
     default: return -1;
     }
 }
@@ -53,6 +60,7 @@ double orc_model_initial_v(int model) {
     case ORC_TT2006: return (double)(-85.23f);
     case ORC_ORD2011: return (double)(-85.f);
     case ORC_MS2003: return 0.00000820413566106744;
+    case ORC_CRN1998: return (double)(-81.180000f); /* courtemanche_ramirez_nattel_1998.h:7 */
     default: return NAN;
     }
 }
@@ -63,6 +71,7 @@ void orc_model_init(int model, double *sv, int64_t n) {
         double *y = sv + i * neq;
         if(model == ORC_TT2006) orc_tt2006_init_cell(y);
         else if(model == ORC_ORD2011) orc_ord2011_init_cell(y);
+        else if(model == ORC_CRN1998) orc_crn1998_init_cell(y);
         else ms_init_cell(y);
     }
 }
@@ -79,6 +88,7 @@ void orc_model_solve_scaled(int model, double *sv, int64_t n, const double *stim
         for(int k = 0; k < num_steps; k++) {
             if(model == ORC_TT2006) orc_tt2006_step_scaled(y, st, dt, s0, s1, s2);
             else if(model == ORC_ORD2011) orc_ord2011_step(y, st, dt);
+            else if(model == ORC_CRN1998) orc_crn1998_step(y, st, dt);
             else ms_step(y, st, dt, s0);
         }
     }
@@ -86,4 +96,16 @@ void orc_model_solve_scaled(int model, double *sv, int64_t n, const double *stim
 
 void orc_model_solve(int model, double *sv, int64_t n, const double *stim, double dt, int num_steps) {
     orc_model_solve_scaled(model, sv, n, stim, 0, dt, num_steps);
+}
+
+int64_t orc_model_solve_adaptive(int model, double *sv, int64_t n, const double *stim, double final_time, double *aux,
+                                 double min_dt, double max_dt, double abs_tol, double rel_tol) {
+    if(model != ORC_CRN1998) return -1;
+    const int neq = orc_model_neq(model);
+    int64_t forced = 0;
+#pragma omp parallel for schedule(static) reduction(+ : forced)
+    for(int64_t i = 0; i < n; i++)
+        forced += orc_adaptive_euler(orc_crn1998_rhs, neq, sv + i * neq, stim ? stim[i] : 0.0, final_time, aux + 3 * i, min_dt,
+                                     max_dt, abs_tol, rel_tol);
+    return forced;
 }

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-enum orc_model { ORC_TT2006 = 0, ORC_ORD2011 = 1, ORC_MS2003 = 2 };
+enum orc_model { ORC_TT2006 = 0, ORC_ORD2011 = 1, ORC_MS2003 = 2, ORC_CRN1998 = 3 };
 
 int    orc_model_neq(int model);
 double orc_model_initial_v(int model);
@@ -36,6 +36,13 @@ void   orc_model_solve(int model, double *sv, int64_t n, const double *stim,
  * GKs; MS2003: tau_close; ignored for ORd); scale == NULL is all 1 */
 void   orc_model_solve_scaled(int model, double *sv, int64_t n, const double *stim,
                               const float *scale, double dt, int num_steps);
+
+/* the adaptive Euler solver (solve_forward_euler_cpu_adpt, default_solvers.c:
+ * 61-176) for n cells to final_time; aux is (n, 3) AoS: dt, time_new,
+ * previous_dt per cell.  Models with a plain RHS only (CRN 1998).  Returns the
+ * number of steps accepted at min_dt with error >= 1, or -1 for another model. */
+int64_t orc_model_solve_adaptive(int model, double *sv, int64_t n, const double *stim, double final_time, double *aux,
+                                 double min_dt, double max_dt, double abs_tol, double rel_tol);
 
 /* ALPHA(beta, cm, dt, dx, dy, dz) of monodomain/constants.h:10-11 */
 double orc_alpha(double beta, double cm, double dt, double dx, double dy, double dz);
@@ -109,6 +116,12 @@ struct orc_tissue {
      * (4 floats per cell, see orc_model_solve_scaled) */
     const uint8_t *stim_mask;
     const float *ionic_scale;
+    /* adaptive Euler ODE solver (ode_solver.c:308-328): each step advances the
+     * cells to cur_time + ode_min_dt with their own step sizes (aux: n x 3
+     * dt, time_new, previous_dt, initialised by orc_tissue_init) */
+    int adaptive;
+    double ode_min_dt, ode_abs_tol, ode_rel_tol;
+    double *aux;
 };
 
 /* set_initial_conditions_fvm (matrix_assembly.c:27-44) + model ICs */

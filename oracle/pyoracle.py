@@ -15,8 +15,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-TT2006, ORD2011, MS2003 = 0, 1, 2
-MODEL_IDS = {"tt2006": TT2006, "ord2011": ORD2011, "ms2003": MS2003}
+TT2006, ORD2011, MS2003, CRN1998 = 0, 1, 2, 3
+MODEL_IDS = {"tt2006": TT2006, "ord2011": ORD2011, "ms2003": MS2003, "crn1998": CRN1998}
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _u8p = C.c_void_p
@@ -42,6 +42,8 @@ def lib() -> C.CDLL:
         L.orc_model_initial_v.argtypes = [C.c_int]
         L.orc_model_init.argtypes = [C.c_int, _dp, C.c_int64]
         L.orc_model_solve.argtypes = [C.c_int, _dp, C.c_int64, C.c_void_p, C.c_double, C.c_int]
+        L.orc_model_solve_adaptive.restype = C.c_int64
+        L.orc_model_solve_adaptive.argtypes = [C.c_int, _dp, C.c_int64, C.c_void_p, C.c_double, _dp] + [C.c_double] * 4
         L.orc_model_solve_scaled.argtypes = [C.c_int, _dp, C.c_int64, C.c_void_p, C.c_void_p, C.c_double, C.c_int]
         L.orc_alpha.restype = C.c_double
         L.orc_alpha.argtypes = [C.c_double] * 6
@@ -102,6 +104,17 @@ def model_solve_scaled(model: int, sv: np.ndarray, stim: np.ndarray | None, scal
     st = None if stim is None else np.ascontiguousarray(stim, dtype=np.float64)
     lib().orc_model_solve_scaled(model, sv.reshape(-1), n, None if st is None else st.ctypes.data, sc.ctypes.data, dt,
                                  num_steps)
+
+
+def model_solve_adaptive(model: int, sv: np.ndarray, stim, final_time: float, aux: np.ndarray, min_dt: float,
+                         max_dt: float, abs_tol: float, rel_tol: float) -> int:
+    """Adaptive Euler to final_time, in place on AoS states (n, neq) and the
+    per-cell (dt, time_new, previous_dt) rows aux (n, 3)."""
+    assert sv.dtype == np.float64 and sv.flags.c_contiguous and aux.dtype == np.float64 and aux.flags.c_contiguous
+    n = sv.shape[0]
+    st = None if stim is None else np.ascontiguousarray(stim, dtype=np.float64)
+    return lib().orc_model_solve_adaptive(model, sv.reshape(-1), n, None if st is None else st.ctypes.data, final_time,
+                                          aux.reshape(-1), min_dt, max_dt, abs_tol, rel_tol)
 
 
 def alpha(beta, cm, dt, dx, dy, dz) -> float:
@@ -180,14 +193,18 @@ class _Tissue(C.Structure):
                 ("active", C.c_void_p), ("sv", C.c_void_p), ("v", C.c_void_p), ("b", C.c_void_p),
                 ("cur_time", C.c_double), ("count", C.c_int64), ("total_cg_its", C.c_int64),
                 ("last_cg_its", C.c_int), ("last_cg_err", C.c_double), ("stim_mask", C.c_void_p),
-                ("ionic_scale", C.c_void_p)]
+                ("ionic_scale", C.c_void_p), ("adaptive", C.c_int), ("ode_min_dt", C.c_double),
+                ("ode_abs_tol", C.c_double), ("ode_rel_tol", C.c_double), ("aux", C.c_void_p)]
 
 
 class Tissue:
     """The oracle's operator-split run on a structured box (AoS state)."""
 
     def __init__(self, shape, h, model, coef27, dt=0.02, beta=0.14, cm=1.0, cg_tol=1e-16, cg_max_its=200,
-                 jacobi=False, stim=(0.0, 2.0, -50.0, 500.0, 0.0), active=None, stim_mask=None, ionic_scale=None):
+                 jacobi=False, stim=(0.0, 2.0, -50.0, 500.0, 0.0), active=None, stim_mask=None, ionic_scale=None,
+                 adaptive=None):
+        """adaptive: None, or (min_dt, abs_tol, rel_tol) for the adaptive Euler
+        ODE solver (max dt = dt)."""
         nx, ny, nz = shape
         n = nx * ny * nz
         self.shape, self.n, self.model = shape, n, model
@@ -210,6 +227,11 @@ class Tissue:
         self.ionic_scale = None if ionic_scale is None else np.ascontiguousarray(ionic_scale, np.float32).reshape(n, 4)
         t.stim_mask = None if self.stim_mask is None else self.stim_mask.ctypes.data
         t.ionic_scale = None if self.ionic_scale is None else self.ionic_scale.ctypes.data
+        self.aux = np.zeros((n, 3), np.float64)
+        if adaptive is not None:
+            t.adaptive = 1
+            t.ode_min_dt, t.ode_abs_tol, t.ode_rel_tol = adaptive
+            t.aux = self.aux.ctypes.data
         self._t = t
         lib().orc_tissue_init(C.byref(t))
 

@@ -13,6 +13,11 @@
  *       from stim_in (nsteps*n doubles, step-major).  Writes the n*neq state
  *       after every record_every steps (default: only the last).
  *       Driver shape: ode_solver.c:235-353 (num_steps = 1, cells_to_solve NULL).
+ *   ionic_adpt <model.so> <n> <nsteps> <dt_pde> <min_dt> <abs_tol> <rel_tol> <states_in|-> <stim_in> <out> [record_every]
+ *       The same with the adaptive Euler solver (ode_solver->adaptive, max_dt =
+ *       dt_pde, ode_solver.c:308-328; num_steps = 1, monodomain_solver.c:510-520):
+ *       each call advances every cell to t + min_dt.  Records the n*neq state
+ *       followed by the per-cell dt, time_new and previous_dt (3*n doubles).
  *   assemble <iso|aniso> <nx> <ny> <nz> <h> <cells_in> <out> [sx sy sz | sl st sn]
  *       Cuboid mesh through set_cuboid_domain_mesh (domain_helpers.c:19-73),
  *       optional deactivation and per-cell fibres from cells_in (per
@@ -101,6 +106,63 @@ static int mode_ionic(int argc, char **argv) {
         solve(s, NULL, (real)t, stim + (size_t)k * n);
         t += dt;
         if((k + 1) % rec == 0) fwrite(s->sv, sizeof(double), (size_t)n * neq, fo);
+    }
+    fclose(fo);
+    printf("neq=%d initial_v=%.17g\n", neq, (double)s->model_data.initial_v);
+    return 0;
+}
+
+static int mode_ionic_adpt(int argc, char **argv) {
+    if(argc < 12) die("ionic_adpt <model.so> <n> <nsteps> <dt_pde> <min_dt> <abs_tol> <rel_tol> <states_in|-> <stim_in> <out> [rec]");
+    const char *lib = argv[2];
+    uint32_t n = (uint32_t)atol(argv[3]);
+    long nsteps = atol(argv[4]);
+    double dt_pde = atof(argv[5]);
+    const char *states_in = argv[9];
+    const char *stim_in = argv[10];
+    const char *out = argv[11];
+    long rec = argc > 12 ? atol(argv[12]) : nsteps;
+
+    void *h = dlopen(lib, RTLD_LAZY);
+    if(!h) die(dlerror());
+    get_cell_model_data_fn *get_data = must_sym(h, "init_cell_model_data");
+    set_ode_initial_conditions_cpu_fn *set_ic = must_sym(h, "set_model_initial_conditions_cpu");
+    solve_model_ode_cpu_fn *solve = must_sym(h, "solve_model_odes_cpu");
+
+    struct ode_solver *s = calloc(1, sizeof(struct ode_solver));
+    get_data(&s->model_data, true, true);
+    int neq = s->model_data.number_of_ode_equations;
+    s->original_num_cells = n;
+    s->num_cells_to_solve = n;
+    s->cells_to_solve = NULL;
+    s->adaptive = true;
+    s->max_dt = (real)dt_pde;
+    s->min_dt = (real)atof(argv[6]);
+    s->abs_tol = (real)atof(argv[7]);
+    s->rel_tol = (real)atof(argv[8]);
+    s->num_steps = 1;
+    set_ic(s, NULL);
+    if(strcmp(states_in, "-") != 0) {
+        double *st = read_doubles(states_in, (size_t)n * (neq + 3));
+        memcpy(s->sv, st, (size_t)n * neq * sizeof(double));
+        memcpy(s->ode_dt, st + (size_t)n * neq, n * sizeof(double));
+        memcpy(s->ode_time_new, st + (size_t)n * (neq + 1), n * sizeof(double));
+        memcpy(s->ode_previous_dt, st + (size_t)n * (neq + 2), n * sizeof(double));
+        free(st);
+    }
+    double *stim = read_doubles(stim_in, (size_t)nsteps * n);
+    FILE *fo = fopen(out, "wb");
+    if(!fo) die("cannot open output");
+    double t = 0.0;
+    for(long k = 0; k < nsteps; k++) {
+        solve(s, NULL, (real)t, stim + (size_t)k * n);
+        t += dt_pde;
+        if((k + 1) % rec == 0) {
+            fwrite(s->sv, sizeof(double), (size_t)n * neq, fo);
+            fwrite(s->ode_dt, sizeof(double), n, fo);
+            fwrite(s->ode_time_new, sizeof(double), n, fo);
+            fwrite(s->ode_previous_dt, sizeof(double), n, fo);
+        }
     }
     fclose(fo);
     printf("neq=%d initial_v=%.17g\n", neq, (double)s->model_data.initial_v);
@@ -227,6 +289,7 @@ static int mode_dirmap(int argc, char **argv) {
 int main(int argc, char **argv) {
     if(argc < 2) die("mode: ionic | assemble | dirmap");
     if(strcmp(argv[1], "ionic") == 0) return mode_ionic(argc, argv);
+    if(strcmp(argv[1], "ionic_adpt") == 0) return mode_ionic_adpt(argc, argv);
     if(strcmp(argv[1], "assemble") == 0) return mode_assemble(argc, argv);
     if(strcmp(argv[1], "dirmap") == 0) return mode_dirmap(argc, argv);
     die("unknown mode");

@@ -170,6 +170,14 @@ void orc_tissue_init(struct orc_tissue *t) {
         t->v[i] = v0;
         t->b[i] = v0 * alpha;
     }
+    /* adaptive solver rows (the models' set_model_initial_conditions_cpu, e.g.
+     * courtemanche_ramirez_nattel_1998.c:12-24): dt = min_dt, the rest 0 */
+    if(t->adaptive && t->aux)
+        for(int64_t i = 0; i < n; i++) {
+            t->aux[3 * i] = t->ode_min_dt;
+            t->aux[3 * i + 1] = 0.0;
+            t->aux[3 * i + 2] = 0.0;
+        }
     t->cur_time = 0.0;
     t->count = 0;
     t->total_cg_its = 0;
@@ -198,7 +206,12 @@ void orc_tissue_step(struct orc_tissue *t) {
     }
     if(t->stim_period > 0.0 && time + t->dt >= t->stim_start + t->stim_period) t->stim_start += t->stim_period;
     /* ODE for the active cells */
-    if(!t->active) {
+    if(t->adaptive) {
+        for(int64_t i = 0; i < n; i++)
+            if(!t->active || t->active[i])
+                orc_model_solve_adaptive(t->model, t->sv + i * neq, 1, stim + i, time + t->ode_min_dt, t->aux + 3 * i,
+                                         t->ode_min_dt, t->dt, t->ode_abs_tol, t->ode_rel_tol);
+    } else if(!t->active) {
         orc_model_solve_scaled(t->model, t->sv, n, stim, t->ionic_scale, t->dt, 1);
     } else {
         for(int64_t i = 0; i < n; i++)

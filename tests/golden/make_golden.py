@@ -81,6 +81,51 @@ def ionic(tmp):
                                      f"{amps.tolist()} for t in [0,2] ms, states every {rec} steps; 200 one-step pairs")
 
 
+def ionic_crn(tmp):
+    """CRN 1998 (a model of the reference's default solvers): G1/G2 for the
+    fixed-step Euler (dt 0.01 ms) and G3 for the adaptive Euler (dt_pde 0.02,
+    min_dt 0.001, tolerances 1e-5): state and (dt, time_new, previous_dt) per
+    cell every 100 calls over 400 ms."""
+    lib = "./shared_libs/libcourtemanche_ramirez_nattel_1998.so"
+    neq = 21
+    amps = np.array([-2000.0, -1200.0])
+    n = len(amps)
+    H_ = os.path.join(REF, "bin", "ref_harness")
+    sp, op = os.path.join(tmp, "stim.bin"), os.path.join(tmp, "out.bin")
+    fx = {"trace_amps": amps}
+    # fixed step
+    dt, nsteps, rec = 0.01, 40000, 100
+    on, t = np.zeros(nsteps, bool), 0.0
+    for k in range(nsteps):
+        on[k] = 0.0 <= t <= 2.0
+        t += dt
+    np.where(on[:, None], amps[None, :], 0.0).tofile(sp)
+    run([H_, "ionic", lib, str(n), str(nsteps), str(dt), "-", sp, op, str(rec)])
+    fx.update(euler_dt=dt, euler_every=rec, euler_trace=np.fromfile(op).reshape(-1, n, neq))
+    run([H_, "ionic", lib, str(n), str(nsteps), str(dt), "-", sp, op, "97"])
+    states = np.fromfile(op).reshape(-1, neq)[:200]
+    s1 = np.where(np.random.default_rng(12).random(len(states)) < 0.5, amps[0], 0.0)
+    st_in = os.path.join(tmp, "states.bin")
+    states.tofile(st_in)
+    s1.tofile(sp)
+    run([H_, "ionic", lib, str(len(states)), "1", str(dt), st_in, sp, op])
+    fx.update(euler_pair_in=states, euler_pair_stim=s1, euler_pair_out=np.fromfile(op).reshape(len(states), neq))
+    # adaptive
+    dtp, min_dt, tol, nsteps, rec = 0.02, 0.001, 1e-5, 20000, 100
+    on = stim_window(nsteps)
+    np.where(on[:, None], amps[None, :], 0.0).tofile(sp)
+    args = [str(dtp), str(min_dt), str(tol), str(tol)]
+    run([H_, "ionic_adpt", lib, str(n), str(nsteps)] + args + ["-", sp, op, str(rec)])
+    recs = np.fromfile(op).reshape(-1, n * (neq + 3))
+    fx.update(adpt_dt_pde=dtp, adpt_min_dt=min_dt, adpt_tol=tol, adpt_every=rec,
+              adpt_trace=recs[:, :n * neq].reshape(-1, n, neq), adpt_aux=recs[:, n * neq:].reshape(-1, 3, n))
+    np.savez_compressed(os.path.join(HERE, "ionic_crn1998.npz"), **fx)
+    PROV["ionic_crn1998.npz"] = (f"ref_harness ionic / ionic_adpt on libcourtemanche_ramirez_nattel_1998.so: "
+                                 f"{n} cells, stimulus amps {amps.tolist()} for t in [0,2] ms; forward Euler dt 0.01 "
+                                 f"(state every 100 steps, 400 ms) and 200 one-step pairs; adaptive Euler dt_pde 0.02, "
+                                 f"min_dt 0.001, abs/rel tol 1e-5 (state + dt/time_new/previous_dt every 100 calls, 400 ms)")
+
+
 def fibres_layered(shape, th0=-60.0, th1=60.0):
     nx, ny, nz = shape
     f = np.zeros((nx * ny * nz, 3))
@@ -271,6 +316,39 @@ def tissue(tmp, name, shape, model, T, every, aniso=False, fibrosis=0.0, keep_ti
                                   f"-30 mV interpolated between saved samples")
 
 
+def tissue_crn_adpt(tmp, name="cable_crn1998_adpt_40", ncell=40, T=100.0, every=20):
+    """Reference MonoAlg3D run of CRN 1998 with the adaptive Euler ODE solver on
+    a cable (as example_configs/cable_mesh_with_CRN_atrial.ini: dt_pde 0.05,
+    adaptive with min dt 0.001, sigma 0.0000176, stimulus -2000 for 2 ms)."""
+    dtp = 0.05
+    out_dir = os.path.join(tmp, name)
+    os.makedirs(out_dir, exist_ok=True)
+    domain = (f"name=Cable Mesh\nstart_dx={H}\nstart_dy={H}\nstart_dz={H}\ncable_length={ncell * H}\n"
+              f"main_function=initialize_grid_with_cable_mesh")
+    asm = "sigma_x=0.0000176\nsigma_y=0.0000176\nsigma_z=0.0000176\nmain_function=homogeneous_sigma_assembly_matrix"
+    ini = ini_text(out_dir, T, "libcourtemanche_ramirez_nattel_1998.so", domain, asm, every)
+    ini = ini.replace(f"dt_pde = {DT}", f"dt_pde = {dtp}").replace(f"[ode_solver]\ndt={DT}", "[ode_solver]\nadaptive=true\ndt=0.001")
+    ini = ini.replace("current = -50.0", "current = -2000.0")
+    path = os.path.join(out_dir, "run.ini")
+    with open(path, "w") as fh:
+        fh.write(ini)
+    log = run([os.path.join(REF, "bin", "MonoAlg3D"), "-c", path])
+    nsteps = int(round(T / dtp)) + 1
+    snaps = read_snapshots(out_dir, (ncell, 1, 1), every, nsteps)
+    lat = np.full(ncell, np.inf)
+    for k in range(1, len(snaps)):
+        a, b = snaps[k - 1], snaps[k]
+        hit = np.isinf(lat) & (a < -30.0) & (b >= -30.0)
+        lat[hit] = (k - 1 + (-30.0 - a[hit]) / (b[hit] - a[hit])) * every * dtp
+    its = [int(line.split("Iterations = ")[1].split(",")[0]) for line in log.splitlines() if line.startswith("t = ")]
+    np.savez_compressed(os.path.join(HERE, f"tissue_{name}.npz"), shape=np.array((ncell, 1, 1)), every=every, dt_pde=dtp,
+                        vm=snaps, lat=lat, cg_iterations_every_1ms=np.array(its))
+    PROV[f"tissue_{name}.npz"] = (f"reference MonoAlg3D (oracle/_ref) cable mesh of {ncell} cells at {H} um, CRN 1998 "
+                                  f"with the adaptive Euler ODE solver (dt_pde {dtp}, min dt 0.001, abs/rel tol 1e-5), "
+                                  f"sigma 0.0000176, stimulus -2000 on x<500 um for t in [0,2] ms, CG tol 1e-16, {T} ms; "
+                                  f"Vm of every cell every {every} steps; LAT = first crossing of -30 mV interpolated")
+
+
 def read_acm(path, shape):
     """The reference's activation_info .acm file (save_mesh.c:754-913): one line
     per active cell, 'cx,cy,cz,hx,hy,hz n [ t1 .. tn ] [ apd1 .. ]'."""
@@ -363,6 +441,9 @@ def main():
     try:
         if "ionic" in groups:
             ionic(tmp)
+        if "crn" in groups:
+            ionic_crn(tmp)
+            tissue_crn_adpt(tmp)
         if "assembly" in groups:
             assembly(tmp)
         if "tissue" in groups:

@@ -199,3 +199,45 @@ def test_reference_core_with_hip_plugins(tmp_path):
     both = np.isfinite(g["lat"])
     assert np.array_equal(np.isfinite(lat), both)
     assert np.abs(lat[both] - g["lat"][both]).max() <= 1e-3
+
+
+def test_reference_core_with_adaptive_crn_plugin(tmp_path):
+    """The reference's MonoAlg3D with [ode_solver] adaptive=true and the CRN
+    1998 plugin over libmonoalg_hip.so (the reference core allocates nothing
+    for the step-size arrays; the plugin does, as the reference's models do):
+    the 40-cell cable run of tissue_cable_crn1998_adpt_40, activation times
+    within 0.05 ms and Vm within 0.05 mV in the median (the accept/reject
+    choices of the step-size control turn rounding into small shifts)."""
+    import subprocess
+    from test_abi import REF_EXE
+    from make_golden import ini_text, read_snapshots
+    plug = os.path.join(os.path.dirname(GOLD), "..", "monoalg3d_c_amd", "plugins")
+    crn = os.path.abspath(os.path.join(plug, "libmono_hip_crn1998.so"))
+    cg = os.path.abspath(os.path.join(plug, "libmono_hip_cg.so"))
+    if not (os.path.exists(crn) and os.path.exists(cg) and os.path.exists(REF_EXE)):
+        pytest.skip("reference build or plugins not built")
+    announce("reference core + HIP adaptive CRN plugin, 40-cell cable 100 ms")
+    g = np.load(os.path.join(GOLD, "tissue_cable_crn1998_adpt_40.npz"))
+    n, every, dtp = 40, int(g["every"]), float(g["dt_pde"])
+    domain = ("name=Cable Mesh\nstart_dx=100.0\nstart_dy=100.0\nstart_dz=100.0\ncable_length=4000.0\n"
+              "main_function=initialize_grid_with_cable_mesh")
+    asm = "sigma_x=0.0000176\nsigma_y=0.0000176\nsigma_z=0.0000176\nmain_function=homogeneous_sigma_assembly_matrix"
+    ini = ini_text(str(tmp_path), 100.0, "libcourtemanche_ramirez_nattel_1998.so", domain, asm, every)
+    ini = ini.replace(f"dt_pde = {DT}", f"dt_pde = {dtp}").replace(f"[ode_solver]\ndt={DT}", "[ode_solver]\nadaptive=true\ndt=0.001")
+    ini = ini.replace("current = -50.0", "current = -2000.0")
+    ini = ini.replace("library_file=shared_libs/libcourtemanche_ramirez_nattel_1998.so", f"library_file={crn}")
+    ini = ini.replace("main_function=conjugate_gradient", f"main_function=mono_conjugate_gradient\nlibrary_file={cg}")
+    path = tmp_path / "run.ini"
+    path.write_text(ini)
+    r = subprocess.run([REF_EXE, "-c", str(path)], cwd=os.path.dirname(os.path.dirname(REF_EXE)), capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    vm = read_snapshots(str(tmp_path), (n, 1, 1), every, int(round(100.0 / dtp)) + 1)
+    assert np.median(np.abs(vm - g["vm"])) < 0.05
+    lat = np.full(n, np.inf)
+    for k in range(1, len(vm)):
+        lo, hi = vm[k - 1], vm[k]
+        hit = np.isinf(lat) & (lo < -30.0) & (hi >= -30.0)
+        lat[hit] = (k - 1 + (-30.0 - lo[hit]) / (hi[hit] - lo[hit])) * every * dtp
+    assert np.array_equal(np.isfinite(lat), np.isfinite(g["lat"]))
+    assert np.abs(lat - g["lat"]).max() < 0.05

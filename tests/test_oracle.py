@@ -85,6 +85,134 @@ def test_ionic_rest_is_steady(oracle):
         assert abs(vs[-1] - vs[99]) < 0.5, (name, vs[99], vs[-1])
 
 
+# ------------------------------------------ CRN 1998, forward / adaptive ----
+# Courtemanche-Ramirez-Nattel 1998 runs on the reference's default solvers
+# (default_solvers.c): plain forward Euler, or the adaptive Euler with per-cell
+# step size.  Fixtures: ionic_crn1998.npz (ref_harness over the compiled
+# reference plugin) and tissue_cable_crn1998_adpt_40.npz (MonoAlg3D run).
+
+def _crn_on(nsteps, dt):
+    on, t = np.zeros(nsteps, bool), 0.0
+    for k in range(nsteps):
+        on[k] = 0.0 <= t <= 2.0
+        t += dt
+    return on
+
+
+def test_crn_euler_trace_and_pairs_bit_exact(oracle):
+    g = _load("ionic_crn1998")
+    mid = oracle.MODEL_IDS["crn1998"]
+    amps, dt, every = g["trace_amps"], float(g["euler_dt"]), int(g["euler_every"])
+    trace = g["euler_trace"]
+    n = len(amps)
+    sv = oracle.model_init(mid, n)
+    on = _crn_on(len(trace) * every, dt)
+    for j in range(len(trace)):
+        for k in range(j * every, (j + 1) * every):
+            oracle.model_solve(mid, sv, amps if on[k] else np.zeros(n), dt, 1)
+        np.testing.assert_array_equal(sv, trace[j], err_msg=f"record {j}")
+    x = np.ascontiguousarray(g["euler_pair_in"])
+    oracle.model_solve(mid, x, g["euler_pair_stim"], dt, 1)
+    np.testing.assert_array_equal(x, g["euler_pair_out"])
+
+
+def _adaptive_run(oracle, g, n_records, min_dt=None, max_dt=None):
+    mid = oracle.MODEL_IDS["crn1998"]
+    amps = g["trace_amps"]
+    n = len(amps)
+    dtp, tol, every = float(g["adpt_dt_pde"]), float(g["adpt_tol"]), int(g["adpt_every"])
+    min_dt = float(g["adpt_min_dt"]) if min_dt is None else min_dt
+    max_dt = dtp if max_dt is None else max_dt
+    sv = oracle.model_init(mid, n)
+    aux = np.zeros((n, 3))
+    aux[:, 0] = min_dt
+    on = _crn_on(n_records * every, dtp)
+    recs, t = [], 0.0
+    for k in range(n_records * every):
+        oracle.model_solve_adaptive(mid, sv, amps if on[k] else np.zeros(n), t + min_dt, aux, min_dt, max_dt, tol, tol)
+        t += dtp
+        if (k + 1) % every == 0:
+            recs.append((sv.copy(), aux.copy()))
+    return recs
+
+
+def test_crn_adaptive_trace_bit_exact(oracle):
+    """400 ms of two stimulated cells: state and the per-cell dt, time_new and
+    previous_dt after every 100 calls, bit for bit (the step-size sequence is
+    the reference's)."""
+    g = _load("ionic_crn1998")
+    recs = _adaptive_run(oracle, g, len(g["adpt_trace"]))
+    for j, (sv, aux) in enumerate(recs):
+        np.testing.assert_array_equal(sv, g["adpt_trace"][j], err_msg=f"record {j}")
+        np.testing.assert_array_equal(aux.T, g["adpt_aux"][j], err_msg=f"record {j}")
+
+
+def test_crn_adaptive_converges_with_tolerance(oracle):
+    """The error control does what it is for: against forward Euler at a 25x
+    smaller fixed step (dt 4e-4 ms, a stand-in for the exact solution) the
+    median |V| error of two 40 ms action potentials shrinks as the tolerance
+    tightens (1e-4 -> 1e-5 -> 1e-6) and is below 0.5 mV at the default 1e-5.
+    (With min dt = max dt the reference's time bookkeeping takes zero-length
+    steps, t + dt - dt rounding, so it is not plain Euler either.)"""
+    mid = oracle.MODEL_IDS["crn1998"]
+    amps = np.array([-2000.0, -1200.0])
+    n = 2
+    dtf, T = 4e-4, 40.0
+    a = oracle.model_init(mid, n)
+    ref, t = [], 0.0
+    per_ms = int(round(1 / dtf))
+    for k in range(int(round(T / dtf))):
+        oracle.model_solve(mid, a, amps if t <= 2.0 else np.zeros(n), dtf, 1)
+        t += dtf
+        if (k + 1) % per_ms == 0:
+            ref.append(a[:, 0].copy())
+    med = []
+    for tol in (1e-4, 1e-5, 1e-6):
+        b = oracle.model_init(mid, n)
+        aux = np.zeros((n, 3))
+        aux[:, 0] = 1e-4
+        got, t = [], 0.0
+        for k in range(int(round(T / 0.02))):
+            oracle.model_solve_adaptive(mid, b, amps if t <= 2.0 else np.zeros(n), t + 1e-4, aux, 1e-4, 0.02, tol, tol)
+            t += 0.02
+            if (k + 1) % 50 == 0:
+                got.append(b[:, 0].copy())
+        med.append(float(np.median(np.abs(np.array(got) - np.array(ref)))))
+    assert med[0] > med[1] > med[2], med
+    assert med[1] < 0.5, med
+
+
+def test_crn_adaptive_tissue_matches_reference(oracle):
+    """Cable of 40 CRN cells with the adaptive solver (dt_pde 0.05, min dt
+    0.001) against MonoAlg3D's run.  The step-size control makes discrete
+    accept/reject choices, so the CG's round-off (different summation order in
+    the reference's Hilbert-ordered cells) shifts single steps near upstrokes:
+    activation times agree within 0.05 ms, Vm to 0.05 mV in the median and
+    within 2 mV at the steepest upstroke samples."""
+    g = _load("tissue_cable_crn1998_adpt_40")
+    n = 40
+    shape = (n, 1, 1)
+    a = oracle.alpha(0.14, 1.0, 0.05, *H)
+    coef = oracle.assemble_iso(shape, H, a, np.full((n, 3), 0.0000176))
+    t = oracle.Tissue(shape, H, oracle.MODEL_IDS["crn1998"], coef, dt=0.05, stim=(0.0, 2.0, -2000.0, 500.0, 0.0),
+                      adaptive=(0.001, 1e-5, 1e-5))
+    every = int(g["every"])
+    vm = [t.v.copy()]
+    for _ in range(len(g["vm"]) - 1):
+        t.step(every)
+        vm.append(t.v.copy())
+    vm = np.array(vm)
+    d = np.abs(vm - g["vm"])
+    assert np.median(d) < 0.05 and d.max() < 2.0, (np.median(d), d.max())
+    lat = np.full(n, np.inf)
+    for k in range(1, len(vm)):
+        lo, hi = vm[k - 1], vm[k]
+        hit = np.isinf(lat) & (lo < -30.0) & (hi >= -30.0)
+        lat[hit] = (k - 1 + (-30.0 - lo[hit]) / (hi[hit] - lo[hit])) * every * 0.05
+    assert np.array_equal(np.isfinite(lat), np.isfinite(g["lat"]))
+    assert np.abs(lat - g["lat"]).max() < 0.05
+
+
 # ------------------------------------------------- ionic scalings --------
 # Per-cell scalings have no reference counterpart (batch sweeps scale TT2006
 # GKr/GCaL/GKs and MS2003 tau_close).  All ones must be the reference model
