@@ -237,6 +237,63 @@ main_function=stim_if_x_less_than
                        f"anisotropic 19-point, {iters} steps from t=0 incl. stimulus; rate = cells*steps/Resolution Time")
 
 
+def run_batch(args) -> dict:
+    """The batch sweep: 256 seeded instances (cables of 100-2000 cells, square
+    sheets of 32-256 cells a side, TT2006 with GKr/GCaL/GKs scaled by U[0.5,1.5]
+    per instance), packed into one atlas, 1 Hz pacing, dt 0.02 ms, CG relative
+    residual 1e-10.  The rate counts instance cells only (not the atlas gaps)."""
+    import monoalg3d_c_amd as m
+    from monoalg3d_c_amd.batch import Batch, sweep_instances
+    insts = sweep_instances(256, args.batch_seed)
+    out = {}
+    for key, prec in (("fp64", m.FP64), ("fp32", m.FP32_REFINE)):
+        if args.only not in (key, "both"):
+            continue
+        t0 = time.time()
+        b = Batch(insts, model=m.MODEL_TT2006, precision=prec, tensor=m.iso_tensor(0.00013), cg_rel_tol=1e-10,
+                  cg_tol=0.0, cg_max_its=500, stim=(0.0, 2.0, -50.0, 500.0, 1000.0))
+        setup_s = time.time() - t0
+        n_cells = sum(i.n for i in insts)
+        b.step(args.warmup)
+        b.sync()
+        s0 = b.stats
+        t0 = time.perf_counter()
+        b.step(args.steps)
+        b.sync()
+        wall = time.perf_counter() - t0
+        s1 = b.stats
+        k2 = min(args.steps, 10)
+        b.tissue.set_timing(True)
+        b.step(k2)
+        b.sync()
+        s2 = b.stats
+        out[key] = dict(rate=n_cells * args.steps / wall, ms_per_step=1e3 * wall / args.steps,
+                        cg_its=(s1["cg_iterations_total"] - s0["cg_iterations_total"]) / args.steps,
+                        inner_its=(s1["inner_iterations_total"] - s0["inner_iterations_total"]) / args.steps,
+                        ionic_ms=(s2["ionic_ms_total"] - s1["ionic_ms_total"]) / k2,
+                        solve_ms=(s2["solve_ms_total"] - s1["solve_ms_total"]) / k2, setup_s=setup_s,
+                        stencil_points=s1["stencil_points"], class_operator=s1["class_operator"])
+        b.close()
+        print(f"[bench] batch {key} {out[key]}", file=sys.stderr, flush=True)
+    from monoalg3d_c_amd.batch import Layout
+    lay = Layout(insts)
+    head = out.get("fp32") or out["fp64"]
+    return {
+        "metric": "cell-timesteps/s, 256-instance batch (TT2006 cables and sheets, FP32+refine)",
+        "value": head["rate"], "unit": "cell-timesteps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if "fp32" in out else "f64", "data": "synthetic",
+        "config": {"workload": "256 instances from seed %d: cables of 100-2000 cells and square sheets of 32-256 "
+                               "cells a side at 100 um, TT2006 with GKr/GCaL/GKs x U[0.5,1.5] per instance, 1 Hz "
+                               "pacing, CG rel. residual 1e-10, dt 0.02 ms" % args.batch_seed,
+                   "instance_cells": sum(i.n for i in insts), "atlas": list(lay.shape),
+                   "atlas_efficiency": round(lay.efficiency(), 4),
+                   "ms_for_500ms_simulated": head["ms_per_step"] * 25000},
+        "fp64": out.get("fp64"), "fp32_refine": out.get("fp32"),
+        "speedup_fp32_refine_vs_fp64": (out["fp32"]["rate"] / out["fp64"]["rate"]) if len(out) == 2 else None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -249,6 +306,9 @@ def main() -> int:
                     help="simulated ms of the untimed FP64 vs FP32+refine activation-time check (0: skip)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run a device copy of this many MiB (counter calibration under rocprofv3 --pmc)")
+    ap.add_argument("--workload", choices=["cuboid", "batch"], default="cuboid",
+                    help="cuboid: the 16M-cell metric; batch: the 256-instance sweep")
+    ap.add_argument("--batch-seed", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is a design "
@@ -258,6 +318,9 @@ def main() -> int:
     if m.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         return 1
+    if args.workload == "batch":
+        print(json.dumps(run_batch(args)))
+        return 0
     shape = tuple(int(s) for s in args.shape.split(","))
     n = shape[0] * shape[1] * shape[2]
     neq = m.model_neq(m.MODEL_ORD2011)
