@@ -173,6 +173,22 @@ int mono_tissue_get_activations(mono_tissue *t, int32_t *n_act, float *act, int3
  * stimulus (n doubles, may be NULL), nsteps ODE steps of dt.  precision selects
  * the FP64 or FP32 kernel. */
 int mono_ionic_run(int model, int precision, double *states, int64_t n, const double *stim, double dt, int nsteps);
+/* Sensor loss and its gradient (no counterpart in the reference; for fitting
+ * conductivity and ionic-parameter fields to recordings).  Runs the tissue of
+ * desc for nsteps steps and records V at the nsens sensor cells every
+ * record_every steps from step 0 (nrec = nsteps/record_every + 1 records,
+ * traces: nrec x nsens, may be NULL).  With target (nrec x nsens) it also
+ * returns loss = sum (traces - target)^2 and, when grad_g / grad_scale are
+ * non-NULL, dloss/dg_cell (n doubles; g_cell = 1 where desc.g_cell is NULL)
+ * and dloss/d ionic_scale[4*i] (n doubles; MS2003 tau_close factor) by the
+ * discrete adjoint of the step: FP64 adjoint solves with the same operator
+ * (symmetric), the forward states recomputed from checkpoints kept every
+ * checkpoint_every steps.  Mitchell-Schaeffer, isotropic uniform tensor,
+ * dt_ode = dt_pde; the forward may be MONO_FP32_REFINE. */
+int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, const int64_t *sensors, int nsens, int record_every,
+                         const double *target, int checkpoint_every, double *traces, double *loss, double *grad_g,
+                         double *grad_scale);
+
 /* Standalone adaptive-Euler solve (solve_model_odes_cpu with ode_solver->adaptive,
  * default_solvers.c:9-46,61-176): advances n cells (states AoS n x neq, in/out)
  * to current_t + min_dt, each from its own time; ode_dt, ode_time_new and

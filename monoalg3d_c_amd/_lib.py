@@ -25,6 +25,7 @@ EXPORTED = (
     "mono_tissue_device_ptrs", "mono_ionic_run", "mono_model_neq", "mono_model_initial_state",
     "mono_model_initial_v", "mono_assemble", "mono_cg_stencil", "mono_cg_csr",
     "mono_tissue_observe_setup", "mono_tissue_observe", "mono_tissue_get_activations", "mono_ionic_run_adaptive",
+    "mono_sensor_gradient",
 )
 
 
@@ -93,6 +94,7 @@ def lib() -> C.CDLL:
     L.mono_tissue_set_timing.argtypes = [vp, C.c_int]
     L.mono_tissue_device_ptrs.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_size_t), C.POINTER(vp)]
     L.mono_ionic_run.argtypes = [C.c_int, C.c_int, vp, i64, vp, C.c_double, C.c_int]
+    L.mono_sensor_gradient.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, vp]
     L.mono_ionic_run_adaptive.argtypes = [C.c_int, C.c_int, vp, i64, vp, C.c_double, vp, vp, vp] + [C.c_double] * 4
     L.mono_model_neq.argtypes = [C.c_int]
     L.mono_model_initial_state.argtypes = [C.c_int, vp]

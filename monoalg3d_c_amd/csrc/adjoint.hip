@@ -1,0 +1,332 @@
+// Sensor loss and its gradient with respect to per-cell conductivity factors
+// and per-cell Mitchell-Schaeffer tau_close factors, by the discrete adjoint of
+// the operator-split step (mono_sensor_gradient in monoalg_hip.h).  No
+// counterpart in the reference.
+//
+// Forward step n -> n+1 of the tissue loop (monodomain_solver.c:616-925 with
+// the Mitchell-Schaeffer model, mitchell_shaeffer_2003.c:56-93, one ODE step
+// per PDE step):
+//   V*      = V_n + dt (-V_n/tau_out + h_n V_n^2 (1 - V_n)/tau_in + stim)
+//   h_{n+1} = h_n + dt ((1 - h_n)/tau_open)            if V_n <  V_gate
+//             h_n + dt (-h_n/(tau_close s))             otherwise
+//   A(g) V_{n+1} = alpha V*
+// Loss: L = sum over records m (every R steps, from step 0) and sensor cells c
+// of (V_{mR}[c] - target[m][c])^2.
+//
+// Backward, with lambda = dL/dV_{n+1} and mu = dL/dh_{n+1} (A is symmetric):
+//   w = A^-1 lambda                                   (adjoint solve, FP64 CG)
+//   dL/dg_k -= sum over faces f of k: dc_f/dg_k (w_k - w_j)(V_k - V_j)   at V_{n+1}
+//   lambda_n = alpha w dV*/dV_n
+//   mu_n     = alpha w dV*/dh_n + mu dh_{n+1}/dh_n
+//   dL/ds_k += mu dh_{n+1}/ds_k
+//   lambda_n += 2 (V_n - target) on the sensors when n is a record step.
+// c_f = 2 sa sb / (sa + sb) * area/distance with sa = sigma g_a (the isotropic
+// face conductivity, assembly_common.c:911-1048).  The MS gate switch has zero
+// derivative almost everywhere and is taken as such.
+//
+// Checkpoints: the forward run keeps (v, states) every K steps; the backward
+// sweep recomputes each K-step segment from its checkpoint, keeping the
+// segment's states, then walks it backwards.  The forward may run in FP32 +
+// refinement; the adjoint always runs in FP64 on the stored states.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/monoalg_hip.h"
+#include "cg.h"
+#include "mono_common.h"
+#include "tissue_impl.h"
+
+namespace mono {
+
+__global__ __launch_bounds__(256) void k_gather(const double *v, const int64_t *idx, int m, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < m) out[i] = v[idx[i]];
+}
+
+// lambda[idx[i]] += val[i] (sensor cells are distinct)
+__global__ __launch_bounds__(256) void k_scatter_add(double *lam, const int64_t *idx, int m, const double *val) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < m) lam[idx[i]] += val[i];
+}
+
+// state row 1 (h) of the tissue in FP64
+template <typename T> __global__ __launch_bounds__(256) void k_copy_row(const T *src, double *dst, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n) dst[i] = (double)src[i];
+}
+
+// Ionic adjoint of one forward-Euler Mitchell-Schaeffer step at (V_n, h_n).
+// lam: in w (the adjoint solve's solution), out lambda_n; mu: in mu_{n+1}, out mu_n.
+__global__ __launch_bounds__(256) void k_ms_adjoint(int64_t n, const uint8_t *active, const double *V, const double *h,
+                                                    double *lam, double *mu, const float *scale, double alpha, double dt,
+                                                    double *grad_s) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    if(active && !active[i]) {
+        lam[i] = 0.0;
+        mu[i] = 0.0;
+        return;
+    }
+    const double tau_in = 0.3, tau_out = 6.0, V_gate = 0.13, tau_open = 120.0;
+    const double s = scale ? (double)scale[4 * i] : 1.0;
+    const double tau_close = 150.0 * s;
+    const double v = V[i], hh = h[i];
+    const double a = alpha * lam[i];
+    const double dVdV = 1.0 + dt * (-1.0 / tau_out + hh * (2.0 * v - 3.0 * v * v) / tau_in);
+    const double dVdh = dt * (v * v * (1.0 - v)) / tau_in;
+    double dhdh, dhds;
+    if(v < V_gate) {
+        dhdh = 1.0 - dt / tau_open;
+        dhds = 0.0;
+    } else {
+        dhdh = 1.0 - dt / tau_close;
+        dhds = dt * hh * 150.0 / (tau_close * tau_close);
+    }
+    const double m1 = mu[i];
+    if(grad_s) grad_s[i] += m1 * dhds;
+    lam[i] = a * dVdV;
+    mu[i] = a * dVdh + m1 * dhdh;
+}
+
+// dL/dg_k -= sum over the six faces of k: dc/dg_k (w_k - w_j)(v_k - v_j)
+__global__ __launch_bounds__(256) void k_grad_g(int nx, int ny, int nz, const uint8_t *active, const double *g, double sx,
+                                                double sy, double sz, double mx, double my, double mz, const double *w,
+                                                const double *v, double *grad_g) {
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(c >= n) return;
+    if(active && !active[c]) return;
+    const int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / ((int64_t)nx * ny));
+    const double gk = g ? g[c] : 1.0;
+    const double wk = w[c], vk = v[c];
+    double acc = 0.0;
+    const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+#pragma unroll
+    for(int f = 0; f < 6; f++) {
+        const int xx = x + off[f][0], yy = y + off[f][1], zz = z + off[f][2];
+        if(xx < 0 || yy < 0 || zz < 0 || xx >= nx || yy >= ny || zz >= nz) continue;
+        const int64_t j = xx + (int64_t)nx * (yy + (int64_t)ny * zz);
+        if(active && !active[j]) continue;
+        const double sig = f < 2 ? sx : (f < 4 ? sy : sz);
+        const double mult = f < 2 ? mx : (f < 4 ? my : mz);
+        const double sa = sig * gk, sb = sig * (g ? g[j] : 1.0);
+        if(sa == 0.0 || sb == 0.0) continue;
+        const double dcdg = mult * sig * 2.0 * sb * sb / ((sa + sb) * (sa + sb));
+        acc += dcdg * (wk - w[j]) * (vk - v[j]);
+    }
+    grad_g[c] -= acc;
+}
+
+}  // namespace mono
+
+using namespace mono;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if(p) (void)hipFree(p);
+    }
+    template <typename T> T *as() const { return (T *)p; }
+};
+
+int check_spec(const mono_tissue_desc *d, int nsteps, const int64_t *sensors, int nsens, int record_every, int checkpoint_every) {
+    if(!d || !sensors || nsens <= 0 || nsteps < 0 || record_every < 1 || checkpoint_every < 1) {
+        mono_set_error("mono_sensor_gradient: bad arguments");
+        return -1;
+    }
+    if(d->model != MONO_MODEL_MS2003 || d->aniso || d->tensor_source != MONO_TENSOR_UNIFORM || d->ode_adaptive) {
+        mono_set_error("mono_sensor_gradient: the adjoint covers Mitchell-Schaeffer on an isotropic uniform tensor "
+                       "(with per-cell factors g) and fixed-step ODEs");
+        return -1;
+    }
+    if(d->dt_ode > 0 && d->dt_ode != d->dt_pde) {
+        mono_set_error("mono_sensor_gradient: one ODE step per PDE step (dt_ode = dt_pde)");
+        return -1;
+    }
+    const int64_t n = (int64_t)d->nx * d->ny * d->nz;
+    for(int i = 0; i < nsens; i++) {
+        if(sensors[i] < 0 || sensors[i] >= n || (d->active && !d->active[sensors[i]])) {
+            mono_set_error("mono_sensor_gradient: sensor %d (cell %lld) outside the active tissue", i, (long long)sensors[i]);
+            return -1;
+        }
+        for(int j = 0; j < i; j++)
+            if(sensors[j] == sensors[i]) {
+                mono_set_error("mono_sensor_gradient: sensor cell %lld listed twice", (long long)sensors[i]);
+                return -1;
+            }
+    }
+    return 0;
+}
+
+struct Checkpoint {
+    DevBuf v, sv;
+    double cur_time = 0.0, stim_start = 0.0;
+    int64_t steps = 0;
+};
+
+int save(mono_tissue *t, Checkpoint &c, size_t sv_bytes) {
+    if(!c.v.p) {
+        MONO_CHECK(hipMalloc(&c.v.p, sizeof(double) * t->n));
+        MONO_CHECK(hipMalloc(&c.sv.p, sv_bytes));
+    }
+    MONO_CHECK(hipMemcpyAsync(c.v.p, t->v, sizeof(double) * t->n, hipMemcpyDeviceToDevice, t->stream));
+    MONO_CHECK(hipMemcpyAsync(c.sv.p, t->sv, sv_bytes, hipMemcpyDeviceToDevice, t->stream));
+    c.cur_time = t->cur_time;
+    c.stim_start = t->stim_start;
+    c.steps = t->st.steps;
+    return 0;
+}
+
+int restore(mono_tissue *t, const Checkpoint &c, size_t sv_bytes) {
+    MONO_CHECK(hipMemcpyAsync(t->v, c.v.p, sizeof(double) * t->n, hipMemcpyDeviceToDevice, t->stream));
+    MONO_CHECK(hipMemcpyAsync(t->sv, c.sv.p, sv_bytes, hipMemcpyDeviceToDevice, t->stream));
+    t->cur_time = c.cur_time;
+    t->st.time = c.cur_time;
+    t->stim_start = c.stim_start;
+    t->st.steps = c.steps;
+    return 0;
+}
+
+// h (state row 1) of the tissue into dst, FP64
+int copy_h(mono_tissue *t, double *dst) {
+    const unsigned grid = (unsigned)((t->n + 255) / 256);
+    if(t->d.precision == MONO_FP64)
+        k_copy_row<double><<<grid, 256, 0, t->stream>>>((const double *)t->sv + t->pitch, dst, t->n);
+    else
+        k_copy_row<float><<<grid, 256, 0, t->stream>>>((const float *)t->sv + t->pitch, dst, t->n);
+    MONO_CHECK(hipGetLastError());
+    return 0;
+}
+
+int record(mono_tissue *t, const DevBuf &idx, int nsens, DevBuf &tmp, double *host_row) {
+    k_gather<<<(nsens + 255) / 256, 256, 0, t->stream>>>(t->v, idx.as<int64_t>(), nsens, tmp.as<double>());
+    MONO_CHECK(hipGetLastError());
+    MONO_CHECK(hipMemcpyAsync(host_row, tmp.p, sizeof(double) * nsens, hipMemcpyDeviceToHost, t->stream));
+    MONO_CHECK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, const int64_t *sensors, int nsens, int record_every,
+                                    const double *target, int checkpoint_every, double *traces, double *loss, double *grad_g,
+                                    double *grad_scale) {
+    if(check_spec(desc, nsteps, sensors, nsens, record_every, checkpoint_every)) return -1;
+    if(mono_device_count() < 1) {
+        mono_set_error("no HIP device available");
+        return -2;
+    }
+    mono_tissue *t = nullptr;
+    if(int rc = mono_tissue_create(desc, &t)) return rc;
+    struct Guard {
+        mono_tissue *t;
+        ~Guard() { mono_tissue_destroy(t); }
+    } guard{t};
+    const int64_t n = t->n;
+    const int nrec = nsteps / record_every + 1;
+    const size_t sv_bytes = (size_t)t->pitch * t->neq * (desc->precision == MONO_FP64 ? 8 : 4);
+    std::vector<double> tr((size_t)nrec * nsens);
+    DevBuf idx, tmp;
+    MONO_CHECK(hipMalloc(&idx.p, sizeof(int64_t) * nsens));
+    MONO_CHECK(hipMalloc(&tmp.p, sizeof(double) * nsens));
+    MONO_CHECK(hipMemcpy(idx.p, sensors, sizeof(int64_t) * nsens, hipMemcpyHostToDevice));
+
+    // forward run with checkpoints
+    const int K = checkpoint_every;
+    const int nseg = nsteps > 0 ? (nsteps + K - 1) / K : 0;
+    std::vector<Checkpoint> ck(nseg);
+    for(int k = 0; k < nsteps; k++) {
+        if(k % K == 0 && save(t, ck[k / K], sv_bytes)) return -1;
+        if(k % record_every == 0 && record(t, idx, nsens, tmp, &tr[(size_t)(k / record_every) * nsens])) return -1;
+        const int rc = mono_tissue_step(t, 1);
+        if(rc < 0) return -1;
+        if(rc == 1) {
+            mono_set_error("mono_sensor_gradient: the forward run stopped on a NaN at step %d", k);
+            return -1;
+        }
+    }
+    if(nsteps % record_every == 0 && record(t, idx, nsens, tmp, &tr[(size_t)(nsteps / record_every) * nsens])) return -1;
+    if(traces) memcpy(traces, tr.data(), sizeof(double) * tr.size());
+    if(!target) return 0;
+
+    // loss and its direct gradient on the sensors, 2 (V - target)
+    std::vector<double> res(tr.size());
+    double L = 0.0;
+    for(size_t i = 0; i < tr.size(); i++) {
+        const double e = tr[i] - target[i];
+        L += e * e;
+        res[i] = 2.0 * e;
+    }
+    if(loss) *loss = L;
+    if(!grad_g && !grad_scale) return 0;
+
+    // backward sweep
+    DevBuf lam, mu, w, gg, gs, dres, Vseg, Hseg;
+    const size_t nb = sizeof(double) * n;
+    MONO_CHECK(hipMalloc(&lam.p, nb));
+    MONO_CHECK(hipMalloc(&mu.p, nb));
+    MONO_CHECK(hipMalloc(&w.p, nb));
+    MONO_CHECK(hipMalloc(&gg.p, nb));
+    MONO_CHECK(hipMalloc(&gs.p, nb));
+    MONO_CHECK(hipMalloc(&dres.p, sizeof(double) * res.size()));
+    MONO_CHECK(hipMalloc(&Vseg.p, nb * (K + 1)));
+    MONO_CHECK(hipMalloc(&Hseg.p, nb * (K + 1)));
+    for(DevBuf *b : {&lam, &mu, &w, &gg, &gs}) MONO_CHECK(hipMemsetAsync(b->p, 0, nb, t->stream));
+    MONO_CHECK(hipMemcpyAsync(dres.p, res.data(), sizeof(double) * res.size(), hipMemcpyHostToDevice, t->stream));
+
+    const mono_tissue_desc &d = t->d;
+    const double sx = desc->tensor[0], sy = desc->tensor[1], sz = desc->tensor[2];  // uniform tensor (host)
+    const double mx = (d.dy * d.dz) / d.dx, my = (d.dx * d.dz) / d.dy, mz = (d.dx * d.dy) / d.dz;
+    CgConfig acfg = t->cgcfg;
+    acfg.mixed = false;
+    acfg.jacobi = 0;
+    acfg.tol = 0.0;
+    acfg.rel_tol = d.cg_rel_tol > 0 ? d.cg_rel_tol : 1e-10;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    const OpView<double> op = t->op.view64();
+    bool live = false;  // lambda, mu non-zero yet
+    for(int seg = nseg - 1; seg >= 0; seg--) {
+        const int k0 = seg * K, k1 = (k0 + K < nsteps) ? k0 + K : nsteps;
+        if(restore(t, ck[seg], sv_bytes)) return -1;
+        double *VS = Vseg.as<double>(), *HS = Hseg.as<double>();
+        MONO_CHECK(hipMemcpyAsync(VS, t->v, nb, hipMemcpyDeviceToDevice, t->stream));
+        if(copy_h(t, HS)) return -1;
+        for(int k = k0; k < k1; k++) {
+            if(mono_tissue_step(t, 1) < 0) return -1;
+            MONO_CHECK(hipMemcpyAsync(VS + (size_t)(k - k0 + 1) * n, t->v, nb, hipMemcpyDeviceToDevice, t->stream));
+            if(copy_h(t, HS + (size_t)(k - k0 + 1) * n)) return -1;
+        }
+        for(int k = k1 - 1; k >= k0; k--) {
+            // lambda += dL/dV_{k+1} on the sensors
+            if((k + 1) % record_every == 0) {
+                k_scatter_add<<<(nsens + 255) / 256, 256, 0, t->stream>>>(lam.as<double>(), idx.as<int64_t>(), nsens,
+                                                                         dres.as<double>() + (size_t)((k + 1) / record_every) * nsens);
+                MONO_CHECK(hipGetLastError());
+                live = true;
+            }
+            if(!live) continue;
+            const double *Vn = VS + (size_t)(k - k0) * n, *Vn1 = VS + (size_t)(k - k0 + 1) * n;
+            const double *Hn = HS + (size_t)(k - k0) * n;
+            // w = A^-1 lambda, warm-started from the previous w
+            CgStats cs{};
+            if(t->cg.solve_fp64(op, lam.as<double>(), w.as<double>(), acfg, &cs, t->stream)) return -1;
+            if(grad_g) {
+                k_grad_g<<<grid, 256, 0, t->stream>>>(d.nx, d.ny, d.nz, t->active, t->g, sx, sy, sz, mx, my, mz, w.as<double>(),
+                                                      Vn1, gg.as<double>());
+                MONO_CHECK(hipGetLastError());
+            }
+            MONO_CHECK(hipMemcpyAsync(lam.p, w.p, nb, hipMemcpyDeviceToDevice, t->stream));
+            k_ms_adjoint<<<grid, 256, 0, t->stream>>>(n, t->active, Vn, Hn, lam.as<double>(), mu.as<double>(), t->ionic_scale,
+                                                      t->alpha, d.dt_pde, gs.as<double>());
+            MONO_CHECK(hipGetLastError());
+        }
+    }
+    MONO_CHECK(hipStreamSynchronize(t->stream));
+    if(grad_g) MONO_CHECK(hipMemcpy(grad_g, gg.p, nb, hipMemcpyDeviceToHost));
+    if(grad_scale) MONO_CHECK(hipMemcpy(grad_scale, gs.p, nb, hipMemcpyDeviceToHost));
+    return 0;
+}

@@ -15,10 +15,11 @@
 
 #include "../../include/monoalg_hip.h"
 #include "cg.h"
-#include "observer.h"
 #include "ionic.h"
 #include "mono_common.h"
+#include "observer.h"
 #include "operator.h"
+#include "tissue_impl.h"
 
 static thread_local char g_err[1024];
 
@@ -31,34 +32,6 @@ void mono_set_error(const char *fmt, ...) {
 
 using namespace mono;
 
-struct mono_tissue {
-    mono_tissue_desc d{};
-    int64_t n = 0;
-    int neq = 0;
-    size_t pitch = 0;
-    void *sv = nullptr;  // double* or float*
-    double *v = nullptr;
-    double *b = nullptr;
-    uint8_t *active = nullptr;
-    double *g = nullptr;
-    double *tensor = nullptr;
-    uint8_t *stim_mask = nullptr;
-    float *ionic_scale = nullptr;
-    double *aux = nullptr;  // adaptive ODE solver: dt, time_new, previous_dt rows of `pitch`
-    Operator op;
-    CgWorkspace cg;
-    CgConfig cgcfg;
-    Observer obs;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    int timing = 0;
-    int num_ode_steps = 1;
-    double dt_ode = 0.02;
-    double alpha = 0.0;
-    double stim_start = 0.0;
-    double cur_time = 0.0;
-    mono_stats st{};
-};
 
 static double alpha_of(const mono_tissue_desc &d) {
     // ALPHA (monodomain/constants.h:10-11); the 1e-8 factor is a float literal there

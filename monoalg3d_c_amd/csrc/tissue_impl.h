@@ -1,0 +1,39 @@
+// The tissue object behind mono_tissue* (monoalg_hip.h), shared by tissue.hip
+// and the sensitivity code (adjoint.hip).
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/monoalg_hip.h"
+#include "cg.h"
+#include "observer.h"
+#include "operator.h"
+
+struct mono_tissue {
+    mono_tissue_desc d{};
+    int64_t n = 0;
+    int neq = 0;
+    size_t pitch = 0;
+    void *sv = nullptr;  // double* or float*
+    double *v = nullptr;
+    double *b = nullptr;
+    uint8_t *active = nullptr;
+    double *g = nullptr;
+    double *tensor = nullptr;
+    uint8_t *stim_mask = nullptr;
+    float *ionic_scale = nullptr;
+    double *aux = nullptr;  // adaptive ODE solver: dt, time_new, previous_dt rows of `pitch`
+    mono::Operator op;
+    mono::CgWorkspace cg;
+    mono::CgConfig cgcfg;
+    mono::Observer obs;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    int timing = 0;
+    int num_ode_steps = 1;
+    double dt_ode = 0.02;
+    double alpha = 0.0;
+    double stim_start = 0.0;
+    double cur_time = 0.0;
+    mono_stats st{};
+};

@@ -113,6 +113,15 @@ def test_device_entry_points_fail_loudly_without_gpu():
     assert L.mono_assemble(C.byref(d), coef.ctypes.data, C.byref(a)) == -2
     with pytest.raises(m.MonoError, match="no HIP device"):
         m.Tissue((4, 4, 2), tensor=m.iso_tensor(0.00013))
+    d2, keep2 = m.make_desc((4, 4, 1), model=m.MODEL_MS2003, tensor=m.iso_tensor(0.00013))
+    sens = np.array([0, 5], np.int64)
+    loss = C.c_double()
+    assert L.mono_sensor_gradient(C.byref(d2), 10, sens.ctypes.data, 2, 5, None, 10, None, C.byref(loss), None, None) == -2
+    assert b"no HIP device" in L.mono_last_error()
+    # argument checks come first: an unsupported model is refused with -1
+    d3, keep3 = m.make_desc((4, 4, 1), model=m.MODEL_TT2006, tensor=m.iso_tensor(0.00013))
+    assert L.mono_sensor_gradient(C.byref(d3), 10, sens.ctypes.data, 2, 5, None, 10, None, C.byref(loss), None, None) == -1
+    assert b"Mitchell-Schaeffer" in L.mono_last_error()
 
 
 def test_product_path_does_not_import_the_oracle():
