@@ -294,6 +294,76 @@ def run_batch(args) -> dict:
     }
 
 
+def run_adjoint(args) -> dict:
+    """The sensitivity workload: a 256 x 256 Mitchell-Schaeffer sheet at 100 um,
+    per-cell conductivity factors ~ log-normal(0, 0.3) and tau_close factors ~
+    U[0.8, 1.2] from seed 1, 64 sensors recording V every 1 ms over 200 ms,
+    squared-error loss against the recordings of the seed-2 fields; FP32 +
+    refinement forward, FP64 adjoint, checkpoints every 50 steps.  Reports the
+    forward time, the loss + gradient time and their ratio, and checks the
+    gradient against central differences (FP64 forward) on a few cells and
+    against the FP64-forward gradient."""
+    import monoalg3d_c_amd as m
+    from monoalg3d_c_amd.adjoint import config5_fields, config5_sensors, sensor_loss_grad, sensor_traces
+    shape = (256, 256, 1)
+    nsteps, every = 10000, 50
+    g, s = config5_fields(shape, seed=1)
+    g2, s2 = config5_fields(shape, seed=2)
+    sens = config5_sensors(shape, 64, seed=5)
+    target = sensor_traces(shape, sens, nsteps, every, g=g2, s=s2)
+    sensor_traces(shape, sens, 200, every, g=g, s=s)  # warm-up (kernel calibration)
+    t0 = time.perf_counter()
+    sensor_traces(shape, sens, nsteps, every, g=g, s=s)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss, dg, ds, _ = sensor_loss_grad(shape, sens, nsteps, every, target, checkpoint_every=50, g=g, s=s)
+    t_grad = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss_all, dg_all, ds_all, _ = sensor_loss_grad(shape, sens, nsteps, every, target, checkpoint_every=0, g=g, s=s)
+    t_grad_all = time.perf_counter() - t0
+    same = float(max(np.linalg.norm(dg_all - dg) / np.linalg.norm(dg), np.linalg.norm(ds_all - ds) / np.linalg.norm(ds)))
+    print(f"[bench] adjoint forward {t_fwd:.2f} s, loss+gradient {t_grad:.2f} s (checkpoints every 50), "
+          f"{t_grad_all:.2f} s (all states kept, gradient rel. diff {same:.1e})", file=sys.stderr, flush=True)
+    # FP64 reference gradient and finite differences (FP64 forward)
+    l64, dg64, ds64, _ = sensor_loss_grad(shape, sens, nsteps, every, target, checkpoint_every=50, g=g, s=s,
+                                          precision=m.FP64)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    fd = []
+    # The MS gate switch (V < V_gate) makes the loss jump where a perturbation
+    # moves some cell's V across V_gate at a step; a central difference that
+    # straddles such a jump is meaningless, so each cell is differenced with
+    # two step sizes and compared only where the two agree to 1e-4.
+    def cdiff(c, rel):
+        gp, gm = g.copy(), g.copy()
+        gp[c] += rel * g[c]
+        gm[c] -= rel * g[c]
+        tp = sensor_traces(shape, sens, nsteps, every, g=gp, s=s, precision=m.FP64)
+        tm = sensor_traces(shape, sens, nsteps, every, g=gm, s=s, precision=m.FP64)
+        return (((tp - target) ** 2).sum() - ((tm - target) ** 2).sum()) / (gp[c] - gm[c])
+    for c in np.argsort(-np.abs(dg64))[:args.adjoint_fd_cells]:
+        d1, d2 = cdiff(c, 1e-4), cdiff(c, 1e-5)
+        ok = abs(d1 - d2) <= 1e-4 * abs(d1)
+        fd.append(dict(cell=int(c), adjoint_fp64=float(dg64[c]), adjoint_fp32_forward=float(dg[c]), central_difference=float(d1),
+                       central_difference_small_step=float(d2), differences_agree=bool(ok),
+                       rel_err_fp64=float(abs(dg64[c] - d1) / abs(d1))))
+        print(f"[bench] fd {fd[-1]}", file=sys.stderr, flush=True)
+    ratio = t_grad / t_fwd
+    return {
+        "metric": "adjoint gradient cost / forward cost, 256x256 MS sheet, 200 ms, 64 sensors",
+        "value": ratio, "unit": "x forward", "n_gpus": 1, "steps": nsteps, "warmup": 200,
+        "ms_per_step": 1e3 * t_grad / nsteps, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 forward / f64 adjoint", "data": "synthetic",
+        "config": {"workload": "256x256 Mitchell-Schaeffer sheet at 100 um, g ~ logN(0,0.3), tau_close x U[0.8,1.2] "
+                               "(seed 1), target from seed 2, 64 sensors every 1 ms over 200 ms, checkpoints every 50 steps"},
+        "forward_s": t_fwd, "loss_and_gradient_s": t_grad, "loss": loss,
+        "all_states_kept": {"loss_and_gradient_s": t_grad_all, "ratio": t_grad_all / t_fwd,
+                            "state_bytes": 16 * 256 * 256 * (nsteps + 1), "gradient_rel_diff_vs_checkpointed": same},
+        "grad_rel_l2_fp32_forward_vs_fp64": {"g": rel(dg, dg64), "tau_close": rel(ds, ds64)},
+        "finite_differences": fd,
+        "fd_max_rel_err": max((f["rel_err_fp64"] for f in fd if f["differences_agree"]), default=None),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -306,8 +376,9 @@ def main() -> int:
                     help="simulated ms of the untimed FP64 vs FP32+refine activation-time check (0: skip)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run a device copy of this many MiB (counter calibration under rocprofv3 --pmc)")
-    ap.add_argument("--workload", choices=["cuboid", "batch"], default="cuboid",
-                    help="cuboid: the 16M-cell metric; batch: the 256-instance sweep")
+    ap.add_argument("--workload", choices=["cuboid", "batch", "adjoint"], default="cuboid",
+                    help="cuboid: the 16M-cell metric; batch: the 256-instance sweep; adjoint: gradient cost")
+    ap.add_argument("--adjoint-fd-cells", type=int, default=4)
     ap.add_argument("--batch-seed", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -320,6 +391,9 @@ def main() -> int:
         return 1
     if args.workload == "batch":
         print(json.dumps(run_batch(args)))
+        return 0
+    if args.workload == "adjoint":
+        print(json.dumps(run_adjoint(args)))
         return 0
     shape = tuple(int(s) for s in args.shape.split(","))
     n = shape[0] * shape[1] * shape[2]

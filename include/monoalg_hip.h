@@ -183,8 +183,9 @@ int mono_ionic_run(int model, int precision, double *states, int64_t n, const do
  * and dloss/d ionic_scale[4*i] (n doubles; MS2003 tau_close factor) by the
  * discrete adjoint of the step: FP64 adjoint solves with the same operator
  * (symmetric), the forward states recomputed from checkpoints kept every
- * checkpoint_every steps.  Mitchell-Schaeffer, isotropic uniform tensor,
- * dt_ode = dt_pde; the forward may be MONO_FP32_REFINE. */
+ * checkpoint_every steps (<= 0: every step's states are kept, 16 bytes per
+ * cell and step, and nothing is recomputed).  Mitchell-Schaeffer, isotropic
+ * uniform tensor, dt_ode = dt_pde; the forward may be MONO_FP32_REFINE. */
 int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, const int64_t *sensors, int nsens, int record_every,
                          const double *target, int checkpoint_every, double *traces, double *loss, double *grad_g,
                          double *grad_scale);

@@ -26,11 +26,16 @@
 //
 // Checkpoints: the forward run keeps (v, states) every K steps; the backward
 // sweep recomputes each K-step segment from its checkpoint, keeping the
-// segment's states, then walks it backwards.  The forward may run in FP32 +
-// refinement; the adjoint always runs in FP64 on the stored states.
+// segment's states, then walks it backwards.  With K <= 0 the forward keeps
+// every step's V and h instead (16 bytes per cell and step) and nothing is
+// recomputed.  The forward may run in FP32 + refinement; the adjoint always
+// runs in FP64 on the stored states.
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
+#include <chrono>
 #include <vector>
 
 #include "../../include/monoalg_hip.h"
@@ -57,23 +62,53 @@ template <typename T> __global__ __launch_bounds__(256) void k_copy_row(const T 
     if(i < n) dst[i] = (double)src[i];
 }
 
-// Ionic adjoint of one forward-Euler Mitchell-Schaeffer step at (V_n, h_n).
-// lam: in w (the adjoint solve's solution), out lambda_n; mu: in mu_{n+1}, out mu_n.
-__global__ __launch_bounds__(256) void k_ms_adjoint(int64_t n, const uint8_t *active, const double *V, const double *h,
-                                                    double *lam, double *mu, const float *scale, double alpha, double dt,
-                                                    double *grad_s) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n) return;
-    if(active && !active[i]) {
-        lam[i] = 0.0;
-        mu[i] = 0.0;
+// One backward step per cell, given w = A^-1 lambda_{n+1} (and the previous
+// solution wprev, for the next solve's extrapolated start wnext):
+//   dL/dg_c -= sum over the six faces of c: dc/dg_c (w_c - w_j)(V_{n+1,c} - V_{n+1,j})
+//   ionic adjoint of the forward-Euler Mitchell-Schaeffer step at (V_n, h_n):
+//   lambda_n = alpha w dV*/dV, mu_n = alpha w dV*/dh + mu dh'/dh, dL/ds += mu dh'/ds
+__global__ __launch_bounds__(256) void k_adjoint_step(int nx, int ny, int nz, const uint8_t *active, const double *g, double sx,
+                                                      double sy, double sz, double mx, double my, double mz, const double *w,
+                                                      const double *wprev, double *wnext, const double *V1, const double *V,
+                                                      const double *h, const float *scale,
+                                                      double alpha, double dt, double *lam, double *mu, double *grad_g,
+                                                      double *grad_s) {
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(c >= n) return;
+    if(active && !active[c]) {
+        lam[c] = 0.0;
+        mu[c] = 0.0;
+        wnext[c] = 0.0;
         return;
     }
+    const double wk = w[c];
+    wnext[c] = 2.0 * wk - wprev[c];  // warm start of the next adjoint solve
+    if(grad_g) {
+        const int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / ((int64_t)nx * ny));
+        const double gk = g ? g[c] : 1.0, vk = V1[c];
+        double acc = 0.0;
+        const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+#pragma unroll
+        for(int f = 0; f < 6; f++) {
+            const int xx = x + off[f][0], yy = y + off[f][1], zz = z + off[f][2];
+            if(xx < 0 || yy < 0 || zz < 0 || xx >= nx || yy >= ny || zz >= nz) continue;
+            const int64_t j = xx + (int64_t)nx * (yy + (int64_t)ny * zz);
+            if(active && !active[j]) continue;
+            const double sig = f < 2 ? sx : (f < 4 ? sy : sz);
+            const double mult = f < 2 ? mx : (f < 4 ? my : mz);
+            const double sa = sig * gk, sb = sig * (g ? g[j] : 1.0);
+            if(sa == 0.0 || sb == 0.0) continue;
+            const double dcdg = mult * sig * 2.0 * sb * sb / ((sa + sb) * (sa + sb));
+            acc += dcdg * (wk - w[j]) * (vk - V1[j]);
+        }
+        grad_g[c] -= acc;
+    }
     const double tau_in = 0.3, tau_out = 6.0, V_gate = 0.13, tau_open = 120.0;
-    const double s = scale ? (double)scale[4 * i] : 1.0;
+    const double s = scale ? (double)scale[4 * c] : 1.0;
     const double tau_close = 150.0 * s;
-    const double v = V[i], hh = h[i];
-    const double a = alpha * lam[i];
+    const double v = V[c], hh = h[c];
+    const double a = alpha * wk;
     const double dVdV = 1.0 + dt * (-1.0 / tau_out + hh * (2.0 * v - 3.0 * v * v) / tau_in);
     const double dVdh = dt * (v * v * (1.0 - v)) / tau_in;
     double dhdh, dhds;
@@ -84,39 +119,10 @@ __global__ __launch_bounds__(256) void k_ms_adjoint(int64_t n, const uint8_t *ac
         dhdh = 1.0 - dt / tau_close;
         dhds = dt * hh * 150.0 / (tau_close * tau_close);
     }
-    const double m1 = mu[i];
-    if(grad_s) grad_s[i] += m1 * dhds;
-    lam[i] = a * dVdV;
-    mu[i] = a * dVdh + m1 * dhdh;
-}
-
-// dL/dg_k -= sum over the six faces of k: dc/dg_k (w_k - w_j)(v_k - v_j)
-__global__ __launch_bounds__(256) void k_grad_g(int nx, int ny, int nz, const uint8_t *active, const double *g, double sx,
-                                                double sy, double sz, double mx, double my, double mz, const double *w,
-                                                const double *v, double *grad_g) {
-    const int64_t n = (int64_t)nx * ny * nz;
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(c >= n) return;
-    if(active && !active[c]) return;
-    const int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / ((int64_t)nx * ny));
-    const double gk = g ? g[c] : 1.0;
-    const double wk = w[c], vk = v[c];
-    double acc = 0.0;
-    const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
-#pragma unroll
-    for(int f = 0; f < 6; f++) {
-        const int xx = x + off[f][0], yy = y + off[f][1], zz = z + off[f][2];
-        if(xx < 0 || yy < 0 || zz < 0 || xx >= nx || yy >= ny || zz >= nz) continue;
-        const int64_t j = xx + (int64_t)nx * (yy + (int64_t)ny * zz);
-        if(active && !active[j]) continue;
-        const double sig = f < 2 ? sx : (f < 4 ? sy : sz);
-        const double mult = f < 2 ? mx : (f < 4 ? my : mz);
-        const double sa = sig * gk, sb = sig * (g ? g[j] : 1.0);
-        if(sa == 0.0 || sb == 0.0) continue;
-        const double dcdg = mult * sig * 2.0 * sb * sb / ((sa + sb) * (sa + sb));
-        acc += dcdg * (wk - w[j]) * (vk - v[j]);
-    }
-    grad_g[c] -= acc;
+    const double m1 = mu[c];
+    if(grad_s) grad_s[c] += m1 * dhds;
+    lam[c] = a * dVdV;
+    mu[c] = a * dVdh + m1 * dhdh;
 }
 
 }  // namespace mono
@@ -134,7 +140,7 @@ struct DevBuf {
 };
 
 int check_spec(const mono_tissue_desc *d, int nsteps, const int64_t *sensors, int nsens, int record_every, int checkpoint_every) {
-    if(!d || !sensors || nsens <= 0 || nsteps < 0 || record_every < 1 || checkpoint_every < 1) {
+    if(!d || !sensors || nsens <= 0 || nsteps < 0 || record_every < 1) {
         mono_set_error("mono_sensor_gradient: bad arguments");
         return -1;
     }
@@ -235,18 +241,38 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
     MONO_CHECK(hipMalloc(&tmp.p, sizeof(double) * nsens));
     MONO_CHECK(hipMemcpy(idx.p, sensors, sizeof(int64_t) * nsens, hipMemcpyHostToDevice));
 
-    // forward run with checkpoints
-    const int K = checkpoint_every;
+    // forward run with checkpoints every K steps, or (checkpoint_every <= 0)
+    // keeping every step's states so that nothing is recomputed
+    const bool store_all = checkpoint_every <= 0;
+    const bool want_grad = target && (grad_g || grad_scale);
+    const int K = store_all ? (nsteps > 0 ? nsteps : 1) : checkpoint_every;
     const int nseg = nsteps > 0 ? (nsteps + K - 1) / K : 0;
-    std::vector<Checkpoint> ck(nseg);
+    std::vector<Checkpoint> ck(store_all ? 0 : nseg);
+    const size_t nb = sizeof(double) * n;
+    DevBuf Vseg, Hseg;  // states of one segment (or of the whole run)
+    if(want_grad) {
+        if(hipMalloc(&Vseg.p, nb * (K + 1)) != hipSuccess || hipMalloc(&Hseg.p, nb * (K + 1)) != hipSuccess) {
+            mono_set_error("mono_sensor_gradient: no device memory for %d steps of states (%.2f GB); use a checkpoint interval",
+                           K + 1, 2.0 * nb * (K + 1) / 1e9);
+            return -1;
+        }
+    }
+    if(want_grad && store_all) {
+        MONO_CHECK(hipMemcpyAsync(Vseg.p, t->v, nb, hipMemcpyDeviceToDevice, t->stream));
+        if(copy_h(t, Hseg.as<double>())) return -1;
+    }
     for(int k = 0; k < nsteps; k++) {
-        if(k % K == 0 && save(t, ck[k / K], sv_bytes)) return -1;
+        if(!store_all && k % K == 0 && save(t, ck[k / K], sv_bytes)) return -1;
         if(k % record_every == 0 && record(t, idx, nsens, tmp, &tr[(size_t)(k / record_every) * nsens])) return -1;
         const int rc = mono_tissue_step(t, 1);
         if(rc < 0) return -1;
         if(rc == 1) {
             mono_set_error("mono_sensor_gradient: the forward run stopped on a NaN at step %d", k);
             return -1;
+        }
+        if(want_grad && store_all) {
+            MONO_CHECK(hipMemcpyAsync(Vseg.as<double>() + (size_t)(k + 1) * n, t->v, nb, hipMemcpyDeviceToDevice, t->stream));
+            if(copy_h(t, Hseg.as<double>() + (size_t)(k + 1) * n)) return -1;
         }
     }
     if(nsteps % record_every == 0 && record(t, idx, nsens, tmp, &tr[(size_t)(nsteps / record_every) * nsens])) return -1;
@@ -265,17 +291,18 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
     if(!grad_g && !grad_scale) return 0;
 
     // backward sweep
-    DevBuf lam, mu, w, gg, gs, dres, Vseg, Hseg;
-    const size_t nb = sizeof(double) * n;
+    DevBuf lam, mu, gg, gs, dres, wb[3];
     MONO_CHECK(hipMalloc(&lam.p, nb));
     MONO_CHECK(hipMalloc(&mu.p, nb));
-    MONO_CHECK(hipMalloc(&w.p, nb));
+    for(DevBuf &b : wb) MONO_CHECK(hipMalloc(&b.p, nb));
     MONO_CHECK(hipMalloc(&gg.p, nb));
     MONO_CHECK(hipMalloc(&gs.p, nb));
     MONO_CHECK(hipMalloc(&dres.p, sizeof(double) * res.size()));
-    MONO_CHECK(hipMalloc(&Vseg.p, nb * (K + 1)));
-    MONO_CHECK(hipMalloc(&Hseg.p, nb * (K + 1)));
-    for(DevBuf *b : {&lam, &mu, &w, &gg, &gs}) MONO_CHECK(hipMemsetAsync(b->p, 0, nb, t->stream));
+    for(DevBuf *b : {&lam, &mu, &gg, &gs, &wb[0], &wb[1], &wb[2]}) MONO_CHECK(hipMemsetAsync(b->p, 0, nb, t->stream));
+    // adjoint solutions rotate through three buffers: the solve of step k
+    // starts from the linear extrapolation 2 w_{k+1} - w_{k+2} of the two
+    // previous solutions, written by the previous step's k_adjoint_step
+    int wi = 0;
     MONO_CHECK(hipMemcpyAsync(dres.p, res.data(), sizeof(double) * res.size(), hipMemcpyHostToDevice, t->stream));
 
     const mono_tissue_desc &d = t->d;
@@ -285,21 +312,38 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
     acfg.mixed = false;
     acfg.jacobi = 0;
     acfg.tol = 0.0;
-    acfg.rel_tol = d.cg_rel_tol > 0 ? d.cg_rel_tol : 1e-10;
+    // adjoint solves to ||r|| <= 1e-8 ||lambda|| (or the forward's relative
+    // tolerance when that is looser): the gradient needs far less than the
+    // forward's accuracy, and each digit costs about two CG iterations
+    acfg.rel_tol = d.cg_rel_tol > 1e-8 ? d.cg_rel_tol : 1e-8;
     const unsigned grid = (unsigned)((n + 255) / 256);
     const OpView<double> op = t->op.view64();
     bool live = false;  // lambda, mu non-zero yet
+    // MONO_TRACE_ADJOINT=1: phase times and adjoint CG iterations on stderr
+    const bool trace = getenv("MONO_TRACE_ADJOINT") != nullptr;
+    double t_re = 0.0, t_adj = 0.0;
+    long adj_its = 0, adj_solves = 0;
+    auto now = [&]() {
+        if(trace) (void)hipStreamSynchronize(t->stream);
+        return std::chrono::steady_clock::now();
+    };
     for(int seg = nseg - 1; seg >= 0; seg--) {
         const int k0 = seg * K, k1 = (k0 + K < nsteps) ? k0 + K : nsteps;
-        if(restore(t, ck[seg], sv_bytes)) return -1;
+        auto c0 = now();
         double *VS = Vseg.as<double>(), *HS = Hseg.as<double>();
-        MONO_CHECK(hipMemcpyAsync(VS, t->v, nb, hipMemcpyDeviceToDevice, t->stream));
-        if(copy_h(t, HS)) return -1;
-        for(int k = k0; k < k1; k++) {
-            if(mono_tissue_step(t, 1) < 0) return -1;
-            MONO_CHECK(hipMemcpyAsync(VS + (size_t)(k - k0 + 1) * n, t->v, nb, hipMemcpyDeviceToDevice, t->stream));
-            if(copy_h(t, HS + (size_t)(k - k0 + 1) * n)) return -1;
+        if(!store_all) {
+            // recompute the segment from its checkpoint, keeping its states
+            if(restore(t, ck[seg], sv_bytes)) return -1;
+            MONO_CHECK(hipMemcpyAsync(VS, t->v, nb, hipMemcpyDeviceToDevice, t->stream));
+            if(copy_h(t, HS)) return -1;
+            for(int k = k0; k < k1; k++) {
+                if(mono_tissue_step(t, 1) < 0) return -1;
+                MONO_CHECK(hipMemcpyAsync(VS + (size_t)(k - k0 + 1) * n, t->v, nb, hipMemcpyDeviceToDevice, t->stream));
+                if(copy_h(t, HS + (size_t)(k - k0 + 1) * n)) return -1;
+            }
         }
+        auto c1 = now();
+        t_re += std::chrono::duration<double>(c1 - c0).count();
         for(int k = k1 - 1; k >= k0; k--) {
             // lambda += dL/dV_{k+1} on the sensors
             if((k + 1) % record_every == 0) {
@@ -313,19 +357,23 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
             const double *Hn = HS + (size_t)(k - k0) * n;
             // w = A^-1 lambda, warm-started from the previous w
             CgStats cs{};
-            if(t->cg.solve_fp64(op, lam.as<double>(), w.as<double>(), acfg, &cs, t->stream)) return -1;
-            if(grad_g) {
-                k_grad_g<<<grid, 256, 0, t->stream>>>(d.nx, d.ny, d.nz, t->active, t->g, sx, sy, sz, mx, my, mz, w.as<double>(),
-                                                      Vn1, gg.as<double>());
-                MONO_CHECK(hipGetLastError());
-            }
-            MONO_CHECK(hipMemcpyAsync(lam.p, w.p, nb, hipMemcpyDeviceToDevice, t->stream));
-            k_ms_adjoint<<<grid, 256, 0, t->stream>>>(n, t->active, Vn, Hn, lam.as<double>(), mu.as<double>(), t->ionic_scale,
-                                                      t->alpha, d.dt_pde, gs.as<double>());
+            double *w = wb[wi].as<double>(), *wprev = wb[(wi + 2) % 3].as<double>(), *wnext = wb[(wi + 1) % 3].as<double>();
+            if(t->cg.solve_fp64(op, lam.as<double>(), w, acfg, &cs, t->stream)) return -1;
+            adj_its += cs.iterations;
+            adj_solves++;
+            k_adjoint_step<<<grid, 256, 0, t->stream>>>(d.nx, d.ny, d.nz, t->active, t->g, sx, sy, sz, mx, my, mz, w, wprev, wnext,
+                                                        Vn1, Vn, Hn, t->ionic_scale, t->alpha, d.dt_pde, lam.as<double>(),
+                                                        mu.as<double>(), grad_g ? gg.as<double>() : nullptr,
+                                                        grad_scale ? gs.as<double>() : nullptr);
             MONO_CHECK(hipGetLastError());
+            wi = (wi + 1) % 3;
         }
+        t_adj += std::chrono::duration<double>(now() - c1).count();
     }
     MONO_CHECK(hipStreamSynchronize(t->stream));
+    if(trace)
+        fprintf(stderr, "[mono] adjoint: recompute %.3f s, adjoint sweep %.3f s, %ld solves, %.2f CG iterations per solve\n",
+                t_re, t_adj, adj_solves, adj_solves ? (double)adj_its / adj_solves : 0.0);
     if(grad_g) MONO_CHECK(hipMemcpy(grad_g, gg.p, nb, hipMemcpyDeviceToHost));
     if(grad_scale) MONO_CHECK(hipMemcpy(grad_scale, gs.p, nb, hipMemcpyDeviceToHost));
     return 0;

@@ -6,6 +6,8 @@
 // status record every few iterations.
 #pragma once
 
+#include <vector>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -67,6 +69,13 @@ class CgWorkspace {
     CgScalars *host_sc = nullptr;   // pinned mirror for polling
     int last_iters = 0;
     int last_inner = 0;
+    // mixed solve: inner iterations of each correction sweep of the last solve
+    // (the next solve enqueues that many sweeps without polling)
+    std::vector<int> last_sched;
+    int *sweep_its = nullptr;  // device: per blind sweep, inner iterations or -1
+    int *host_its = nullptr;   // pinned mirror
 };
+
+constexpr int kMaxBlindSweeps = 8;
 
 }  // namespace mono

@@ -610,6 +610,12 @@ __global__ __launch_bounds__(kBlock) void k_update(OpView<TC> op, TX *__restrict
     write_partials<2>(v, Pout);
 }
 
+// inner iterations run by the sweep that just ended, or -1 when the outer
+// solve had already converged (its inner and correction kernels were empty)
+__global__ void k_note_sweep(const CgScalars *si, const CgScalars *so, int *out) {
+    if(threadIdx.x == 0) out[0] = so->done ? -1 : si->iters - 1;
+}
+
 __global__ __launch_bounds__(kBlock) void k_correct(int64_t n, double *__restrict__ x, const float *__restrict__ d32,
                                                     const CgScalars *so) {
     if(so->done) return;
@@ -818,6 +824,8 @@ int CgWorkspace::init(int64_t n_, int64_t) {
     MONO_CHECK(hipMemset(sc, 0, sizeof(CgScalars)));
     MONO_CHECK(hipMemset(sc_in, 0, sizeof(CgScalars)));
     MONO_CHECK(hipHostMalloc(&host_sc, 2 * sizeof(CgScalars), hipHostMallocDefault));
+    MONO_CHECK(hipMalloc(&sweep_its, sizeof(int) * kMaxBlindSweeps));
+    MONO_CHECK(hipHostMalloc(&host_its, sizeof(int) * kMaxBlindSweeps, hipHostMallocDefault));
     return 0;
 }
 
@@ -831,6 +839,11 @@ int CgWorkspace::release() {
     if(sc) MONO_CHECK(hipFree(sc));
     if(sc_in) MONO_CHECK(hipFree(sc_in));
     if(host_sc) MONO_CHECK(hipHostFree(host_sc));
+    if(sweep_its) MONO_CHECK(hipFree(sweep_its));
+    if(host_its) MONO_CHECK(hipHostFree(host_its));
+    sweep_its = nullptr;
+    host_its = nullptr;
+    last_sched.clear();
     r = p = p2 = q = z = P0 = P1 = nullptr;
     r32 = p32 = p32b = q32 = d32 = nullptr;
     sc = sc_in = host_sc = nullptr;
@@ -898,7 +911,9 @@ int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x
     MONO_CHECK(hipGetLastError());
     const double *rz = cfg.jacobi ? z : r;
     int enq = 0;
-    int chunk = cfg.poll_chunk > 0 ? cfg.poll_chunk : (last_iters > 2 ? last_iters - 1 : 2);
+    // the last solve's iteration count includes the kernel that detected
+    // convergence: enqueue that many, so one poll usually sees the solve done
+    int chunk = cfg.poll_chunk > 0 ? cfg.poll_chunk : (last_iters > 2 ? last_iters : 2);
     for(;;) {
         for(int j = 0; j < chunk; j++) {
             const int k = enq + j;
@@ -935,11 +950,10 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
     const int grid = grid_for(op64.n);
     if(check_pattern(op64.pat)) return -1;
     float *pb[2] = {p32, p32b};
-    int inner_total = 0;
-    int chunk0 = last_inner > 2 ? last_inner - 1 : 4;
+    int inner_total = 0, corrections = 0;
+    int chunk0 = last_inner > 2 ? last_inner + 1 : 4;
     CgScalars *h_out = host_sc, *h_in = host_sc + 1;
-    for(int sweep = 0;; sweep++) {
-        const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, 0, cfg.max_outer, sweep};
+    auto residual = [&](int sweep, const CgSetup &cs) -> int {
         // outer residual in FP64 -> r32 = p32 = r, d32 = 0; r.r, b.b -> P0
         int nres = 0;
         with_op(op64, [&](auto C_, auto P_) {
@@ -947,42 +961,83 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
             nres = launch_stencil<double, double, C, P, false>(op64, x, nullptr, nullptr, EpiResOuter{b, r32, pb[0], d32}, cs, -1, 0,
                                                                nullptr, 0, P0, sc, nullptr, s);
         });
+        return nres;
+    };
+    auto inner = [&](const CgSetup &cs, int k, int nres) -> int {
+        float *pin = pb[k & 1], *pout = pb[(k + 1) & 1];
+        int nst = 0;
+        with_op(op32, [&](auto C_, auto P_) {
+            constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
+            nst = launch_stencil<float, float, C, P, true>(op32, pin, r32, pout, EpiSpmv<float>{q32}, cs, k == 0 ? 2 : 1, k, P0,
+                                                           k == 0 ? nres : grid, P1, sc_in, sc, s);
+        });
+        if(nst < 0) return -1;
+        k_update<float, float, float><<<grid, kBlock, 0, s>>>(op32, d32, pout, q32, r32, nullptr, 0, k, P1, nst, P0, sc_in);
+        return 0;
+    };
+    // Sweeps enqueued without polling, as many as the previous solve needed,
+    // each with the inner iterations it took plus two.  Kernels of a finished
+    // (inner or outer) solve exit on the device flags, so a schedule that is
+    // too long costs empty launches, one that is too short a partial
+    // correction; either way the polled sweeps below finish the solve to the
+    // same outer test.  k_note_sweep records what each blind sweep did.
+    const int blind = cfg.poll_chunk > 0 ? 0 : (int)last_sched.size();
+    for(int sweep = 0; sweep < blind; sweep++) {
+        const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, 0, cfg.max_outer, sweep};
+        const int nres = residual(sweep, cs);
+        if(nres < 0) return -1;
+        for(int k = 0; k < last_sched[sweep] + 2; k++)
+            if(inner(cs, k, nres)) return -1;
+        k_note_sweep<<<1, 64, 0, s>>>(sc_in, sc, sweep_its + sweep);
+        k_correct<<<grid, kBlock, 0, s>>>(op64.n, x, d32, sc);
+        MONO_CHECK(hipGetLastError());
+    }
+    std::vector<int> sched;
+    bool blind_read = blind == 0;
+    if(blind > 0) chunk0 = 1;  // the next sweep mostly confirms convergence
+    for(int sweep = blind;; sweep++) {
+        const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, 0, cfg.max_outer, sweep};
+        const int nres = residual(sweep, cs);
         if(nres < 0) return -1;
         MONO_CHECK(hipGetLastError());
         int chunk = cfg.poll_chunk > 0 ? cfg.poll_chunk : chunk0;
         int enq = 0;
         for(;;) {
-            for(int j = 0; j < chunk; j++) {
-                const int k = enq + j;
-                float *pin = pb[k & 1], *pout = pb[(k + 1) & 1];
-                int nst = 0;
-                with_op(op32, [&](auto C_, auto P_) {
-                    constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
-                    nst = launch_stencil<float, float, C, P, true>(op32, pin, r32, pout, EpiSpmv<float>{q32}, cs, k == 0 ? 2 : 1, k, P0,
-                                                                   k == 0 ? nres : grid, P1, sc_in, sc, s);
-                });
-                if(nst < 0) return -1;
-                k_update<float, float, float><<<grid, kBlock, 0, s>>>(op32, d32, pout, q32, r32, nullptr, 0, k, P1, nst, P0, sc_in);
-            }
+            for(int j = 0; j < chunk; j++)
+                if(inner(cs, enq + j, nres)) return -1;
             MONO_CHECK(hipGetLastError());
             enq += chunk;
             MONO_CHECK(hipMemcpyAsync(h_out, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
             MONO_CHECK(hipMemcpyAsync(h_in, sc_in, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+            if(!blind_read) MONO_CHECK(hipMemcpyAsync(host_its, sweep_its, sizeof(int) * blind, hipMemcpyDeviceToHost, s));
             MONO_CHECK(hipStreamSynchronize(s));
+            if(!blind_read) {
+                // inner iterations of the blind sweeps; -1 marks a sweep after convergence
+                for(int k = 0; k < blind && host_its[k] >= 0; k++) {
+                    sched.push_back(host_its[k]);
+                    inner_total += host_its[k];
+                    corrections++;
+                }
+                blind_read = true;
+            }
             if(h_in->done || enq > cfg.max_its + 2) break;
             chunk = cfg.poll_chunk > 0 ? cfg.poll_chunk : 2;
         }
         if(h_out->done) break;
         inner_total += h_in->iters - 1;
+        sched.push_back(h_in->iters - 1);
+        corrections++;
         if(sweep == 0) last_inner = h_in->iters - 1;
         k_correct<<<grid, kBlock, 0, s>>>(op64.n, x, d32, sc);
         MONO_CHECK(hipGetLastError());
         chunk0 = 2;
     }
+    if(sched.size() > (size_t)kMaxBlindSweeps) sched.resize(kMaxBlindSweeps);
+    last_sched = sched;
     if(st) {
         st->iterations = inner_total + 1;
         st->error = h_out->rr_true;
-        st->outer = h_out->iters;
+        st->outer = corrections + 1;
         st->inner_total = inner_total;
     }
     return 0;

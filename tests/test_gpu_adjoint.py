@@ -99,8 +99,9 @@ def test_fp32_forward_gradient_close_to_fp64():
 
 
 def test_checkpoint_interval_does_not_change_gradient():
-    """Recomputing from checkpoints every 7 steps or keeping one segment gives
-    the same gradient bit for bit (the forward kernels are deterministic)."""
+    """Recomputing from checkpoints every 7 steps, keeping one segment, or
+    keeping every step's states without recomputation gives the same gradient
+    bit for bit (the forward kernels are deterministic)."""
     import monoalg3d_c_amd as m
     from monoalg3d_c_amd.adjoint import sensor_loss_grad, sensor_traces
     announce("adjoint checkpoint interval 7 vs 1000")
@@ -108,9 +109,11 @@ def test_checkpoint_interval_does_not_change_gradient():
     target = sensor_traces(SHAPE, sensors, 200, EVERY, g=g2, s=s2, precision=m.FP64, stim=STIM)
     a = sensor_loss_grad(SHAPE, sensors, 200, EVERY, target, checkpoint_every=7, g=g, s=s, precision=m.FP64, stim=STIM)
     b = sensor_loss_grad(SHAPE, sensors, 200, EVERY, target, checkpoint_every=1000, g=g, s=s, precision=m.FP64, stim=STIM)
-    assert a[0] == b[0]
-    np.testing.assert_array_equal(a[1], b[1])
-    np.testing.assert_array_equal(a[2], b[2])
+    c = sensor_loss_grad(SHAPE, sensors, 200, EVERY, target, checkpoint_every=0, g=g, s=s, precision=m.FP64, stim=STIM)
+    for other in (b, c):
+        assert a[0] == other[0]
+        np.testing.assert_array_equal(a[1], other[1])
+        np.testing.assert_array_equal(a[2], other[2])
 
 
 def test_torch_autograd_function():
