@@ -253,10 +253,12 @@ def lat_from(snaps, every, thr=-30.0):
     return lat
 
 
-def tissue(tmp, name, shape, model, T, every, aniso=False, fibrosis=0.0, keep_times=(), cable=False):
+def case_files(out_dir, shape, model, T, every, aniso=False, fibrosis=0.0, cable=False):
+    """Write the reference inputs of a tissue case into out_dir: run.ini and,
+    for the anisotropic cases, a custom mesh (fibrotic cells omitted, seed 13)
+    and a fibre file.  Returns (ini path, active mask, per-cell fibres or None)."""
     nx, ny, nz = shape
     n = nx * ny * nz
-    out_dir = os.path.join(tmp, name)
     os.makedirs(out_dir, exist_ok=True)
     active = np.ones(n, np.uint8)
     if cable:
@@ -294,6 +296,14 @@ def tissue(tmp, name, shape, model, T, every, aniso=False, fibrosis=0.0, keep_ti
     ini = os.path.join(out_dir, "run.ini")
     with open(ini, "w") as fh:
         fh.write(ini_text(out_dir, T, lib, domain, asm, every))
+    return ini, active, fib
+
+
+def tissue(tmp, name, shape, model, T, every, aniso=False, fibrosis=0.0, keep_times=(), cable=False):
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    out_dir = os.path.join(tmp, name)
+    ini, active, fib = case_files(out_dir, shape, model, T, every, aniso=aniso, fibrosis=fibrosis, cable=cable)
     log = run([os.path.join(REF, "bin", "MonoAlg3D"), "-c", ini])
     nsteps = int(round(T / DT)) + 1
     snaps = read_snapshots(out_dir, shape, every, nsteps)

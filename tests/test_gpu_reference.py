@@ -241,3 +241,32 @@ def test_reference_core_with_adaptive_crn_plugin(tmp_path):
         lat[hit] = (k - 1 + (-30.0 - lo[hit]) / (hi[hit] - lo[hit])) * every * dtp
     assert np.array_equal(np.isfinite(lat), np.isfinite(g["lat"]))
     assert np.abs(lat - g["lat"]).max() < 0.05
+
+
+def test_reference_ini_drop_in(tmp_path):
+    """The reference's own configuration for the fibrotic ORd case (custom mesh
+    omitting 10% of the cells, fibre file, anisotropic assembly) loaded by
+    monoalg3d_c_amd.inputs and run on the device, writing V_it_<k>.bin as the
+    reference does: the same snapshots as the reference's run within 1e-5 mV,
+    activation times within 1e-3 ms."""
+    from make_golden import case_files, read_snapshots
+    from monoalg3d_c_amd.inputs import load_ini, run
+    announce("reference .ini drop-in: fibrotic ORd 24x24x6 custom mesh + fibres, 10 ms")
+    g = np.load(os.path.join(GOLD, "tissue_fibrotic_ord_24x24x6.npz"))
+    shape = tuple(int(s) for s in g["shape"])
+    every = int(g["every"])
+    ini, act, fib = case_files(str(tmp_path), shape, "ord2011", 10.0, every, aniso=True, fibrosis=0.1)
+    np.testing.assert_array_equal(act, g["active"])
+    sim = load_ini(ini)
+    st = run(sim)
+    assert st["nan_detected"] == 0 and st["class_operator"] == 0
+    snaps = read_snapshots(str(tmp_path), shape, every, len(g["probe_vm"]) * every - every + 1)
+    live = act.astype(bool)
+    assert np.abs(snaps[:, g["probe_cells"]][:, live[g["probe_cells"]]] - g["probe_vm"][:, live[g["probe_cells"]]]).max() < 1e-5
+    for tms in (5, 10):
+        k = int(round(tms / (every * DT)))
+        assert np.abs(snaps[k][live] - g[f"vm_t{tms}"][live]).max() < 1e-5
+    lat = lat_from(snaps, every)
+    both = live & np.isfinite(g["lat"])
+    assert np.array_equal(np.isfinite(lat[live]), np.isfinite(g["lat"][live]))
+    assert np.abs(lat[both] - g["lat"][both]).max() <= 1e-3
