@@ -132,3 +132,31 @@ def test_batch_fp32_refine_activation_times():
         assert np.array_equal(np.isfinite(l64), np.isfinite(l32)), i
         fin = np.isfinite(l64)
         assert np.abs(l64[fin] - l32[fin]).max() <= 0.5, i
+
+
+def test_sweep_file_runs_match_standalone_runs(tmp_path):
+    """A sweep file in the reference's [batch]/[modify] format over cable
+    length and the GKr factor: the four runs go into one atlas, and each run's
+    V_it files equal a stand-alone run of that cable (FP64, 5 ms)."""
+    import monoalg3d_c_amd as m
+    from make_golden import case_files, read_snapshots
+    from monoalg3d_c_amd.sweep import plan, run_sweep
+    announce("sweep file: 2 cable lengths x 2 GKr factors, one atlas")
+    ini, _, _ = case_files(str(tmp_path / "base"), (300, 1, 1), "tt2006", 5.0, 50, cable=True)
+    b = tmp_path / "sweep.ini"
+    b.write_text("[batch]\ninitial_config=" + ini + "\noutput_folder=" + str(tmp_path / "out") +
+                 "\nnum_simulations_per_parameter_change=1\n[modify]\n"
+                 "domain|cable_length=list|10000|15000\nextra_data|GKr_multiplicator=list|0.5|1.5\n")
+    assert len(plan(str(b))) == 1
+    done = run_sweep(str(b))
+    assert [d[0] for d in done] == [1, 2, 3, 4]
+    for (num_run, out), (length, gkr) in zip(done, [(100, 0.5), (100, 1.5), (150, 0.5), (150, 1.5)]):
+        snaps = read_snapshots(out, (length, 1, 1), 50, 251)
+        t = m.Tissue((length, 1, 1), precision=m.FP64, cg_tol=1e-16, tensor=m.iso_tensor(0.00013),
+                     ionic_scale=np.tile(np.array([gkr, 1, 1, 1], np.float32), (length, 1)))
+        want = [t.v]
+        for _ in range(len(snaps) - 1):
+            t.step(50)
+            want.append(t.v)
+        t.close()
+        np.testing.assert_allclose(snaps, np.array(want), rtol=0, atol=VM_TOL, err_msg=f"run {num_run}")
