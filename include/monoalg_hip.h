@@ -154,6 +154,23 @@ int mono_tissue_set_timing(mono_tissue *t, int on);
  * `pitch` elements, FP64 or FP32 by precision), b (FP64, n) */
 int mono_tissue_device_ptrs(mono_tissue *t, void **v, void **sv, size_t *pitch, void **b);
 
+/* Checkpoint and restart: SAVE_STATE save_simulation_state
+ * (save_state_config.h, save_state.c:16-155) and RESTORE_STATE
+ * restore_simulation_state (restore_state_config.h, restore_state.c:45-256).
+ * save writes grid_checkpoint.dat, monodomain_solver_checkpoint.dat and
+ * ode_solver_checkpoint.dat in the reference's layout (states of the active
+ * cells, gpu = 1 state-major, real = double) into the existing directory dir,
+ * plus hip_solver_checkpoint.dat (exact stimulus start, solver schedule,
+ * statistics, activation observer) with which a restart repeats the
+ * uninterrupted run bit for bit.  restore reads the reference's files
+ * (matching cells by centre, either state layout, double or float states) and
+ * the fourth file when present; the tissue must have the same shape, spacing,
+ * model, dt_pde, active cells and ODE solver kind.  Without the fourth file the
+ * stimulus restarts from the configured start, as the reference's restart
+ * does.  Adaptive-mesh checkpoints (a cells-to-solve list) are refused. */
+int mono_tissue_save_state(mono_tissue *t, const char *dir);
+int mono_tissue_restore_state(mono_tissue *t, const char *dir);
+
 /* Activation times and APDs (save_with_activation_times, save_mesh.c:730-913),
  * kept on the device.  Call mono_tissue_observe where the reference calls its
  * save function: every print_rate iterations, before mono_tissue_step, the
@@ -168,6 +185,8 @@ int mono_tissue_observe(mono_tissue *t);
 /* n_act, n_apd: n counts; act, apd: max_events x n floats (row k = k-th event,
  * NaN where absent).  Any pointer may be NULL. */
 int mono_tissue_get_activations(mono_tissue *t, int32_t *n_act, float *act, int32_t *n_apd, float *apd);
+/* max_events of the observer (set up, or restored with a checkpoint); 0 when none */
+int mono_tissue_observe_events(const mono_tissue *t);
 
 /* Standalone ionic solve: n cells, states AoS (n x neq, FP64, in/out), per-cell
  * stimulus (n doubles, may be NULL), nsteps ODE steps of dt.  precision selects

@@ -12,6 +12,7 @@ All compute runs in libmonoalg_hip.so (HIP, gfx950).
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -184,6 +185,17 @@ class Tissue:
         assert a.shape == (self.neq, self.n)
         check(lib().mono_tissue_set_state(self._h, ptr(a)), "mono_tissue_set_state")
 
+    def save_state(self, directory: str) -> None:
+        """Checkpoint into directory (created if missing): the reference's three
+        checkpoint files plus this library's own (save_state.c:16-155)."""
+        os.makedirs(directory, exist_ok=True)
+        check(lib().mono_tissue_save_state(self._h, os.fsencode(directory)), "mono_tissue_save_state")
+
+    def restore_state(self, directory: str) -> None:
+        """Restart from a checkpoint written here or by the reference
+        (restore_state.c:45-256)."""
+        check(lib().mono_tissue_restore_state(self._h, os.fsencode(directory)), "mono_tissue_restore_state")
+
     @property
     def stats(self) -> dict:
         s = Stats()
@@ -202,14 +214,13 @@ class Tissue:
                       max_events=8, vm_double=False) -> None:
         check(lib().mono_tissue_observe_setup(self._h, activation_threshold, apd_threshold, time_threshold,
                                               max_events, int(vm_double)), "mono_tissue_observe_setup")
-        self._max_events = max_events
 
     def observe(self) -> None:
         """One save call at the current time (every print_rate steps, before stepping)."""
         check(lib().mono_tissue_observe(self._h), "mono_tissue_observe")
 
     def activations(self) -> dict:
-        n, k = self.n, self._max_events
+        n, k = self.n, lib().mono_tissue_observe_events(self._h)
         n_act = np.zeros(n, np.int32)
         n_apd = np.zeros(n, np.int32)
         act = np.zeros((k, n), np.float32)

@@ -21,10 +21,10 @@ TENSOR_UNIFORM, TENSOR_PER_LAYER, TENSOR_PER_CELL = 0, 1, 2
 EXPORTED = (
     "mono_last_error", "mono_device_count", "mono_tissue_create", "mono_tissue_destroy", "mono_tissue_step",
     "mono_tissue_sync", "mono_tissue_num_cells", "mono_tissue_neq", "mono_tissue_get_v", "mono_tissue_set_v",
-    "mono_tissue_get_state", "mono_tissue_set_state", "mono_tissue_get_stats", "mono_tissue_set_timing",
+    "mono_tissue_get_state", "mono_tissue_set_state", "mono_tissue_save_state", "mono_tissue_restore_state", "mono_tissue_get_stats", "mono_tissue_set_timing",
     "mono_tissue_device_ptrs", "mono_ionic_run", "mono_model_neq", "mono_model_initial_state",
     "mono_model_initial_v", "mono_assemble", "mono_cg_stencil", "mono_cg_csr",
-    "mono_tissue_observe_setup", "mono_tissue_observe", "mono_tissue_get_activations", "mono_ionic_run_adaptive",
+    "mono_tissue_observe_setup", "mono_tissue_observe", "mono_tissue_get_activations", "mono_tissue_observe_events", "mono_ionic_run_adaptive",
     "mono_sensor_gradient",
 )
 
@@ -90,6 +90,8 @@ def lib() -> C.CDLL:
     L.mono_tissue_set_v.argtypes = [vp, vp]
     L.mono_tissue_get_state.argtypes = [vp, vp]
     L.mono_tissue_set_state.argtypes = [vp, vp]
+    L.mono_tissue_save_state.argtypes = [vp, C.c_char_p]
+    L.mono_tissue_restore_state.argtypes = [vp, C.c_char_p]
     L.mono_tissue_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.mono_tissue_set_timing.argtypes = [vp, C.c_int]
     L.mono_tissue_device_ptrs.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_size_t), C.POINTER(vp)]
@@ -106,6 +108,7 @@ def lib() -> C.CDLL:
     L.mono_tissue_observe_setup.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int]
     L.mono_tissue_observe.argtypes = [vp]
     L.mono_tissue_get_activations.argtypes = [vp, vp, vp, vp, vp]
+    L.mono_tissue_observe_events.argtypes = [vp]
     L.mono_cg_csr.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
     _lib = L
     return L

@@ -172,6 +172,10 @@ struct Checkpoint {
     DevBuf v, sv;
     double cur_time = 0.0, stim_start = 0.0;
     int64_t steps = 0;
+    // the mixed solve's sweep schedule and polling sizes: a recomputed segment
+    // then repeats the forward run's solves exactly
+    std::vector<int> sched;
+    int last_iters = 0, last_inner = 0;
 };
 
 int save(mono_tissue *t, Checkpoint &c, size_t sv_bytes) {
@@ -184,6 +188,9 @@ int save(mono_tissue *t, Checkpoint &c, size_t sv_bytes) {
     c.cur_time = t->cur_time;
     c.stim_start = t->stim_start;
     c.steps = t->st.steps;
+    c.sched = t->cg.last_sched;
+    c.last_iters = t->cg.last_iters;
+    c.last_inner = t->cg.last_inner;
     return 0;
 }
 
@@ -194,6 +201,9 @@ int restore(mono_tissue *t, const Checkpoint &c, size_t sv_bytes) {
     t->st.time = c.cur_time;
     t->stim_start = c.stim_start;
     t->st.steps = c.steps;
+    t->cg.last_sched = c.sched;
+    t->cg.last_iters = c.last_iters;
+    t->cg.last_inner = c.last_inner;
     return 0;
 }
 

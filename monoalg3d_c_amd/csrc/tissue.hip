@@ -9,6 +9,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <vector>
@@ -345,6 +346,8 @@ int mono_tissue_get_activations(mono_tissue *t, int32_t *n_act, float *act, int3
     return 0;
 }
 
+int mono_tissue_observe_events(const mono_tissue *t) { return (t && t->obs.last_v) ? t->obs.max_events : 0; }
+
 int64_t mono_tissue_num_cells(const mono_tissue *t) { return t ? t->n : -1; }
 int mono_tissue_neq(const mono_tissue *t) { return t ? t->neq : -1; }
 
@@ -423,7 +426,13 @@ int mono_tissue_step(mono_tissue *t, int nsteps) {
             if((tt >= t->stim_start) && (tt <= t->stim_start + d.stim_duration)) amp += d.stim_current;
             tt += t->dt_ode;
         }
-        if(d.stim_period > 0.0 && tt >= t->stim_start + d.stim_period) t->stim_start = t->stim_start + d.stim_period;
+        // the shifted start goes back into the configuration as "%lf" text and
+        // is parsed again (ode_solver.c:224-230): it keeps six decimals
+        if(d.stim_period > 0.0 && tt >= t->stim_start + d.stim_period) {
+            char buf[64];
+            snprintf(buf, sizeof(buf), "%lf", t->stim_start + d.stim_period);
+            t->stim_start = strtod(buf, nullptr);
+        }
 
         if(t->timing) MONO_CHECK(hipEventRecord(t->ev[0], t->stream));
         if(launch_ionic(t, amp)) return -1;

@@ -193,6 +193,9 @@ class Simulation:
     binary: bool = False
     activation: dict = field(default_factory=dict)   # save_with_activation_times parameters
     active: np.ndarray | None = None
+    checkpoint_dir: str | None = None   # [save_state] output_dir
+    checkpoint_rate: int = 0            # [save_state] save_rate (0: at the end only)
+    restore_dir: str | None = None      # [restore_state] input_dir
 
     @property
     def n(self) -> int:
@@ -313,6 +316,23 @@ def load_ini(path: str) -> Simulation:
             sim.activation = dict(time_threshold=num(sv.get("time_threshold", 10.0)),
                                   activation_threshold=num(sv.get("activation_threshold", -30.0)),
                                   apd_threshold=num(sv.get("apd_threshold", -83.0)))
+    # checkpoints (monodomain_solver.c:231-260,555-562): both directories
+    # default to the [save_result] output directory
+    ss = cfg.get("save_state")
+    if ss is not None:
+        if ss.get("main_function", "save_simulation_state") != "save_simulation_state":
+            raise ConfigError(f"[save_state] main_function {ss['main_function']} is not supported")
+        sim.checkpoint_dir = rel(ss["output_dir"]) if "output_dir" in ss else sim.output_dir
+        sim.checkpoint_rate = int(num(ss.get("save_rate", 0)))
+        if not sim.checkpoint_dir:
+            raise ConfigError("[save_state] needs an output_dir")
+    rs = cfg.get("restore_state")
+    if rs is not None:
+        if rs.get("main_function", "restore_simulation_state") != "restore_simulation_state":
+            raise ConfigError(f"[restore_state] main_function {rs['main_function']} is not supported")
+        sim.restore_dir = rel(rs["input_dir"]) if "input_dir" in rs else sim.output_dir
+        if not sim.restore_dir:
+            raise ConfigError("[restore_state] needs an input_dir")
     return sim
 
 
@@ -353,7 +373,16 @@ def run(sim: Simulation | str, precision: int = FP64, max_events: int = 8, progr
         if saving and act:
             t.observe_setup(sim.activation["activation_threshold"], sim.activation["apd_threshold"],
                             sim.activation["time_threshold"], max_events)
-        for k in range(sim.nsteps):
+        k, cur = 0, 0.0
+        if sim.restore_dir:
+            t.restore_state(sim.restore_dir)
+            st = t.stats
+            k, cur = st["steps"], st["time"]
+        if sim.checkpoint_dir:
+            os.makedirs(sim.checkpoint_dir, exist_ok=True)
+        # while (cur_time - finalT <= dt_pde), count and time carried over a
+        # restart (monodomain_solver.c:582-616,906-933)
+        while cur - sim.simulation_time <= sim.dt_pde:
             if saving and k % sim.print_rate == 0:
                 if act:
                     t.observe()
@@ -363,6 +392,12 @@ def run(sim: Simulation | str, precision: int = FP64, max_events: int = 8, progr
                 break
             if progress:
                 progress(k)
+            k += 1
+            cur += sim.dt_pde
+            if sim.checkpoint_dir and sim.checkpoint_rate and k % sim.checkpoint_rate == 0:
+                t.save_state(sim.checkpoint_dir)
+        if sim.checkpoint_dir and sim.checkpoint_rate == 0:
+            t.save_state(sim.checkpoint_dir)
         if saving and act:
             a = t.activations()
             write_acm(os.path.join(sim.output_dir, "activation_info_it_0.acm"), sim.shape, sim.h, a["n_act"], a["act"],

@@ -239,6 +239,20 @@ class Tissue:
         for _ in range(k):
             lib().orc_tissue_step(C.byref(self._t))
 
+    def restore(self, ck: dict) -> None:
+        """Continue from a checkpoint read by read_reference_checkpoint, as
+        restore_simulation_state leaves the solver (restore_state.c:45-256):
+        potentials and states of the active cells, current time and iteration;
+        the stimulus start stays the configured one."""
+        have = ~np.isnan(ck["v"])
+        self.v[have] = ck["v"][have]
+        sv = self.sv.reshape(self.n, self.neq)
+        sv[have] = ck["sv"][have]
+        if ck["aux"] is not None and self._t.adaptive:
+            self.aux[have] = ck["aux"][have]
+        self._t.cur_time = ck["time"]
+        self._t.count = ck["iteration"]
+
     @property
     def time(self) -> float:
         return self._t.cur_time
@@ -250,6 +264,58 @@ class Tissue:
     @property
     def total_cg_its(self) -> int:
         return self._t.total_cg_its
+
+
+def read_reference_checkpoint(grid: bytes, solver: bytes, ode: bytes, shape, h, neq_: int) -> dict:
+    """The three files of save_simulation_state (save_state.c:16-155) on a
+    structured box: v (n,) and sv (n, neq) in lexicographic cell order (NaN
+    for cells the checkpoint does not hold), aux (n, 3) adaptive rows or None,
+    time and iteration.  Fields are written one by one without padding; the
+    k-th active grid record owns state column k (sv_position = grid_position,
+    monodomain_solver.c:1116); gpu = 1 files are state-major [rows][N], gpu = 0
+    files cell-major [N][neq]."""
+    import struct
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    # grid: mesh header (6 doubles, 2 uint32), then per cell record
+    nc, na = struct.unpack_from("<II", grid, 48)
+    off, lex, vals = 56, [], []
+    for _ in range(nc):
+        cx, cy, cz, v = struct.unpack_from("<4d", grid, off)
+        active = grid[off + 8 * 11 + 1]
+        (extra,) = struct.unpack_from("<Q", grid, off + 8 * 11 + 2)
+        off += 98 + extra
+        if active:
+            i, j, k = (int(np.floor(c / d)) for c, d in ((cx, h[0]), (cy, h[1]), (cz, h[2])))
+            assert (i + 0.5) * h[0] == cx and (j + 0.5) * h[1] == cy and (k + 0.5) * h[2] == cz
+            lex.append(i + nx * (j + ny * k))
+            vals.append(v)
+    assert off == len(grid) and len(lex) == na
+    lex = np.array(lex)
+    # monodomain solver: struct (88 bytes) + time_info {current_t, final_t, dt, iteration}
+    assert len(solver) == 120
+    time = struct.unpack_from("<d", solver, 88)[0]
+    (iteration,) = struct.unpack_from("<i", solver, 112)
+    # ODE solver header
+    adaptive = ode[0]
+    (ncts,) = struct.unpack_from("<Q", ode, 33)
+    flag, gpu = ode[41], ode[42]
+    assert flag == 0
+    (N,) = struct.unpack_from("<I", ode, 55)
+    assert N == na
+    rows = neq_ + (3 if adaptive and gpu else 0)
+    body = np.frombuffer(ode[59:], np.float64)
+    assert body.size == rows * N
+    cols = body.reshape(rows, N) if gpu else body.reshape(N, neq_).T
+    v = np.full(n, np.nan)
+    v[lex] = vals
+    sv = np.full((n, neq_), np.nan)
+    sv[lex] = cols[:neq_].T
+    aux = None
+    if rows > neq_:
+        aux = np.full((n, 3), np.nan)
+        aux[lex] = cols[neq_:].T
+    return {"v": v, "sv": sv, "aux": aux, "time": time, "iteration": iteration, "adaptive": bool(adaptive), "gpu": bool(gpu)}
 
 
 class Observer:

@@ -47,10 +47,12 @@ PLUG_ten_tusscher_2006              := models_library/ten_tusscher/ten_tusscher_
 PLUG_ohara_rudy_endo_2011           := models_library/Ohara_Rudy/Ohara_Rudy_2011.c
 PLUG_mitchell_shaeffer_2003         := models_library/mitchell_shaeffer/mitchell_shaeffer_2003.c
 PLUG_courtemanche_ramirez_nattel_1998 := models_library/courtemanche_ramirez_nattel_1998/courtemanche_ramirez_nattel_1998.c
+PLUG_default_save_state             := save_state_library/save_state.c
+PLUG_default_restore_state          := restore_state_library/restore_state.c
 
 PLUGINS := default_stimuli default_domains default_matrix_assembly default_linear_system_solver \
            default_save_mesh default_update_monodomain ten_tusscher_2006 ohara_rudy_endo_2011 \
-           mitchell_shaeffer_2003 courtemanche_ramirez_nattel_1998
+           mitchell_shaeffer_2003 courtemanche_ramirez_nattel_1998 default_save_state default_restore_state
 
 PLUGIN_SOS := $(addprefix $(LIBDIR)/lib,$(addsuffix .so,$(PLUGINS)))
 

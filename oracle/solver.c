@@ -3,6 +3,7 @@
  * TEST INFRASTRUCTURE ONLY.
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include "oracle.h"
@@ -204,7 +205,14 @@ void orc_tissue_step(struct orc_tissue *t) {
             if(t->stim_mask ? t->stim_mask[i] != 0 : cx < t->stim_x_limit) stim[i] += t->stim_current;
         }
     }
-    if(t->stim_period > 0.0 && time + t->dt >= t->stim_start + t->stim_period) t->stim_start += t->stim_period;
+    /* the shifted start is written back into the configuration as "%lf" text
+     * and parsed again (ode_solver.c:224-230), so it keeps six decimals */
+    const double sub_dt = t->adaptive ? t->ode_min_dt : t->dt;
+    if(t->stim_period > 0.0 && time + sub_dt >= t->stim_start + t->stim_period) {
+        char buf[64];
+        snprintf(buf, sizeof(buf), "%lf", t->stim_start + t->stim_period);
+        t->stim_start = strtod(buf, NULL);
+    }
     /* ODE for the active cells */
     if(t->adaptive) {
         for(int64_t i = 0; i < n; i++)

@@ -359,6 +359,63 @@ def tissue_crn_adpt(tmp, name="cable_crn1998_adpt_40", ncell=40, T=100.0, every=
                                   f"Vm of every cell every {every} steps; LAT = first crossing of -30 mV interpolated")
 
 
+CK_SHAPE, CK_T1, CK_T2, CK_STIM = (10, 8, 2), 6.0, 12.0, (0.0, 0.5, -50.0, 500.0, 1.5)
+
+
+def checkpoint_case(out_dir, T, extra=""):
+    """A 10x8x2 TT2006 box with a periodic stimulus (every 1.5 ms for 0.5 ms)
+    and Vm saved every 50 steps; extra is appended to the .ini."""
+    nx, ny, nz = CK_SHAPE
+    os.makedirs(out_dir, exist_ok=True)
+    domain = (f"name=Cuboid Mesh\nstart_dx={H}\nstart_dy={H}\nstart_dz={H}\nside_length_x={nx * H}\n"
+              f"side_length_y={ny * H}\nside_length_z={nz * H}\nmain_function=initialize_grid_with_cuboid_mesh")
+    asm = "sigma_x=0.00013\nsigma_y=0.00013\nsigma_z=0.00013\nmain_function=homogeneous_sigma_assembly_matrix"
+    text = ini_text(out_dir, T, MODELS["tt2006"][0], domain, asm, 50)
+    text = text.replace("duration = 2.0\n", f"duration = {CK_STIM[1]}\nperiod = {CK_STIM[4]}\n")
+    ini = os.path.join(out_dir, "run.ini")
+    with open(ini, "w") as fh:
+        fh.write(text + extra)
+    return ini
+
+
+def read_iters(out_dir, shape, iters):
+    nx, ny, nz = shape
+    out = np.full((len(iters), nx * ny * nz), np.nan)
+    for r, k in enumerate(iters):
+        rec = np.fromfile(os.path.join(out_dir, f"V_it_{k}.bin")).reshape(-1, 7)
+        lex = (np.floor(rec[:, 0] / H) + nx * (np.floor(rec[:, 1] / H) + ny * np.floor(rec[:, 2] / H))).astype(int)
+        out[r, lex] = rec[:, 6]
+    return out
+
+
+def checkpoint(tmp):
+    """The reference's checkpoint (save_simulation_state at the end of a 6 ms
+    run), its restart to 12 ms (restore_simulation_state) and its uninterrupted
+    12 ms run."""
+    ck = os.path.join(tmp, "ck")
+    os.makedirs(ck, exist_ok=True)
+    bin_ = os.path.join(REF, "bin", "MonoAlg3D")
+    run([bin_, "-c", checkpoint_case(os.path.join(tmp, "ck_a"), CK_T1,
+                                      f"[save_state]\nmain_function=save_simulation_state\noutput_dir={ck}\n")])
+    run([bin_, "-c", checkpoint_case(os.path.join(tmp, "ck_b"), CK_T2,
+                                      f"[restore_state]\nmain_function=restore_simulation_state\ninput_dir={ck}\n")])
+    run([bin_, "-c", checkpoint_case(os.path.join(tmp, "ck_c"), CK_T2)])
+    iters = np.arange(350, 601, 50)
+    fx = {"shape": np.array(CK_SHAPE), "stim": np.array(CK_STIM), "iters": iters,
+          "restart_vm": read_iters(os.path.join(tmp, "ck_b"), CK_SHAPE, iters),
+          "continuous_vm": read_iters(os.path.join(tmp, "ck_c"), CK_SHAPE, iters),
+          "checkpoint_vm": read_iters(os.path.join(tmp, "ck_a"), CK_SHAPE, [300])[0]}
+    for name in ("grid", "monodomain_solver", "ode_solver"):
+        fx[f"file_{name}"] = np.fromfile(os.path.join(ck, f"{name}_checkpoint.dat"), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "checkpoint_tt2006_10x8x2.npz"), **fx)
+    PROV["checkpoint_tt2006_10x8x2.npz"] = (
+        f"reference MonoAlg3D (oracle/_ref) 10x8x2 cuboid at {H} um, tt2006, isotropic 0.13 S/m, stimulus -50 on "
+        f"x<500 um for 0.5 ms every 1.5 ms; save_simulation_state at the end of a {CK_T1} ms run (file_* = the three "
+        f"checkpoint files as bytes), restore_simulation_state from it to {CK_T2} ms (restart_vm), the same "
+        f"{CK_T2} ms uninterrupted (continuous_vm); Vm at iterations `iters`, lexicographic cells; checkpoint_vm = "
+        f"iteration 300 of the first run")
+
+
 def read_acm(path, shape):
     """The reference's activation_info .acm file (save_mesh.c:754-913): one line
     per active cell, 'cx,cy,cz,hx,hy,hz n [ t1 .. tn ] [ apd1 .. ]'."""
@@ -468,6 +525,8 @@ def main():
             # instances of a batch, each run on its own by the reference
             tissue(tmp, "cable_tt2006_300", (300, 1, 1), "tt2006", 60.0, 50, cable=True, keep_times=(20, 60))
             tissue(tmp, "sheet_tt2006_48x32", (48, 32, 1), "tt2006", 40.0, 50, keep_times=(20, 40))
+        if "checkpoint" in groups:
+            checkpoint(tmp)
         if "config1" in groups:
             tissue(tmp, "config1_tt2006_64x64x16", (64, 64, 16), "tt2006", 50.0, 10, keep_times=(20,))
     finally:

@@ -98,18 +98,21 @@ def test_fp32_forward_gradient_close_to_fp64():
         assert np.linalg.norm(a - b) <= 1e-2 * np.linalg.norm(b)
 
 
-def test_checkpoint_interval_does_not_change_gradient():
+@pytest.mark.parametrize("prec_name", ["FP64", "FP32_REFINE"])
+def test_checkpoint_interval_does_not_change_gradient(prec_name):
     """Recomputing from checkpoints every 7 steps, keeping one segment, or
     keeping every step's states without recomputation gives the same gradient
-    bit for bit (the forward kernels are deterministic)."""
+    bit for bit (the forward kernels are deterministic, and a checkpoint keeps
+    the mixed solve's sweep schedule)."""
     import monoalg3d_c_amd as m
     from monoalg3d_c_amd.adjoint import sensor_loss_grad, sensor_traces
-    announce("adjoint checkpoint interval 7 vs 1000")
+    prec = getattr(m, prec_name)
+    announce(f"adjoint checkpoint interval 7 vs 1000 vs all, {prec_name}")
     g, s, g2, s2, sensors = _case()
     target = sensor_traces(SHAPE, sensors, 200, EVERY, g=g2, s=s2, precision=m.FP64, stim=STIM)
-    a = sensor_loss_grad(SHAPE, sensors, 200, EVERY, target, checkpoint_every=7, g=g, s=s, precision=m.FP64, stim=STIM)
-    b = sensor_loss_grad(SHAPE, sensors, 200, EVERY, target, checkpoint_every=1000, g=g, s=s, precision=m.FP64, stim=STIM)
-    c = sensor_loss_grad(SHAPE, sensors, 200, EVERY, target, checkpoint_every=0, g=g, s=s, precision=m.FP64, stim=STIM)
+    a = sensor_loss_grad(SHAPE, sensors, 200, EVERY, target, checkpoint_every=7, g=g, s=s, precision=prec, stim=STIM)
+    b = sensor_loss_grad(SHAPE, sensors, 200, EVERY, target, checkpoint_every=1000, g=g, s=s, precision=prec, stim=STIM)
+    c = sensor_loss_grad(SHAPE, sensors, 200, EVERY, target, checkpoint_every=0, g=g, s=s, precision=prec, stim=STIM)
     for other in (b, c):
         assert a[0] == other[0]
         np.testing.assert_array_equal(a[1], other[1])
