@@ -218,8 +218,16 @@ template <typename TC> __device__ __forceinline__ double diag_of(const OpView<TC
 // p = rz + beta * p_in (written for the block's own cells to p_out, a buffer
 // distinct from p_in, so halo reads never see updated values).
 // ---------------------------------------------------------------------------
-constexpr int kTX = 64, kTY = 8, kW = kTX + 2, kH = kTY + 2, kPL = kW * kH;
+#ifndef MONO_TILE_X
+#define MONO_TILE_X 64
+#endif
+#ifndef MONO_TILE_Y
+#define MONO_TILE_Y 8
+#endif
+constexpr int kTX = MONO_TILE_X, kTY = MONO_TILE_Y, kW = kTX + 2, kH = kTY + 2, kPL = kW * kH;
+static_assert(kBlock % kTX == 0 && (kTX * kTY) % kBlock == 0 && 2 * kTY <= kWave, "tile shape");
 constexpr int kNLD = (kPL + kBlock - 1) / kBlock;
+constexpr int kPLS = kNLD * kBlock;  // LDS plane stride: every lane of every load set has a slot
 constexpr int kRPT = kTX * kTY / kBlock;  // rows per thread
 
 struct TileGeom {
@@ -227,12 +235,6 @@ struct TileGeom {
     int64_t nitems, per;   // (tile, plane) items and items per block
     int lo[3], hi[3];      // interior range of each axis (extent 1: its cell; extent 2: empty)
 };
-
-template <typename TV, bool FUSE>
-__device__ __forceinline__ TV staged(const TV *__restrict__ x, const TV *__restrict__ rz, TV beta, int64_t c) {
-    if constexpr(FUSE) return rz[c] + beta * x[c];
-    else return x[c];
-}
 
 template <typename TC> using scalar_ptr = const __attribute__((address_space(4))) TC *;
 
@@ -274,8 +276,8 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
     // CM: 0 per-cell rows, 1 class table, 2 code table (7-point rows)
     constexpr bool CLS = CM == 1;
     static_assert(CM != 2 || PAT == 7, "code tables hold 7-point rows");
-    __shared__ TV pl[3 * kPL];
-    __shared__ TC crows[CLS ? 9 * 27 : 1];
+    __shared__ TV pl[3 * kPLS];
+    __shared__ TC crows[CLS ? kBlock : 1];  // 9 x 27 class rows, padded to one slot per thread
     __shared__ TC ctab[CM == 2 ? kCodes * PAT : 1];
     if constexpr(CM == 2) {
         // the code table (identical in every segment: rewriting it is benign);
@@ -296,39 +298,60 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
         goff[j] = gok[j] ? gx + op.nx * gy : 0;
     }
     // two register sets keep the loads of planes z+2 and z+3 in flight while
-    // plane z is computed
-    TV ra[kNLD], rb[kNLD];
-    auto fetch = [&](TV(&reg)[kNLD], int z) {
-        const bool zin = z >= 0 && z < op.nz;
-        const int64_t zo = sz * (zin ? z : 0);
+    // plane z is computed.  They hold the raw loaded words: the direction
+    // update p = r + beta p and the zero padding are applied when the plane is
+    // put into LDS, so no arithmetic on a loaded value (which would make the
+    // wave wait for the load) sits between a fetch and its put.
+    struct Fetched {
+        TV a[kNLD], b[FUSE ? kNLD : 1];
+        bool zin;
+    };
+    Fetched ra, rb;
+    // Every load of the plane loop is issued unconditionally (a plane that is
+    // not needed reads element 0 of the tile's first row), so the number of
+    // loads in flight is the same on every path and the compiler's wait counts
+    // can leave the next planes' loads outstanding.
+    // (the scheduling barriers keep each plane's loads together and in
+    // program order, which the wait counts rely on)
+    auto fetch = [&](Fetched &f, int z, bool need) {
+        f.zin = need && z >= 0 && z < op.nz;
+        const int64_t zo = sz * (f.zin ? z : 0);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for(int j = 0; j < kNLD; j++) {
-            const TV val = staged<TV, FUSE>(x, rz, beta, zo + goff[j]);  // always a valid address
-            reg[j] = (zin && gok[j]) ? val : TV(0);
+            const int64_t c = need ? zo + goff[j] : 0;  // always a valid address
+            f.a[j] = x[c];
+            if constexpr(FUSE) f.b[j] = rz[c];
         }
+        __builtin_amdgcn_sched_barrier(0);
     };
-    auto put = [&](const TV(&reg)[kNLD], int slot) {
+    auto put = [&](const Fetched &f, int slot) {
 #pragma unroll
-        for(int j = 0; j < kNLD; j++)
-            if(tid + j * kBlock < kPL) pl[slot * kPL + tid + j * kBlock] = reg[j];
+        for(int j = 0; j < kNLD; j++) {
+            TV val;
+            if constexpr(FUSE) val = f.b[j] + beta * f.a[j];
+            else val = f.a[j];
+            pl[slot * kPLS + tid + j * kBlock] = (f.zin && gok[j]) ? val : TV(0);  // lanes past kPL fill padding
+        }
     };
     // class mode: the 9 class rows of a plane (27 slots each) are staged in LDS,
     // one plane ahead, for tiles on an x or y face and on the boundary planes
     const bool face_xy = CLS && (x0 < g.lo[0] || x0 + kTX - 1 > g.hi[0] || y0 < g.lo[1] || y0 + kTY - 1 > g.hi[1]);
     auto need_rows = [&](int z) { return CLS && (face_xy || z < g.lo[2] || z > g.hi[2]); };
     TC rrow = TC(0);
-    auto fetch_rows = [&](int z) {
-        if(tid < 9 * 27 && need_rows(z)) rrow = op.table[(int64_t)z * 9 * 27 + tid];
+    auto fetch_rows = [&](int z, bool need) {
+        if constexpr(CLS) rrow = op.table[(need && tid < 9 * 27 && need_rows(z)) ? (int64_t)z * 9 * 27 + tid : 0];
     };
     const int tx = tid % kTX, ty = tid / kTX;
     const int gx = x0 + tx;
     const bool gx_in = gx < op.nx;
     // class mode: lanes on an x face are evaluated by the face pass below
     const bool xf = !CLS || (gx >= g.lo[0] && gx <= g.hi[0]);
-    // face pass: lanes 0..15 of wave 0 take the tile's x-face cells (row l & 7,
-    // left face for l < 8 when the tile holds x = 0, right face otherwise)
-    const int fx = tid < 8 ? 0 : op.nx - 1;
-    const bool face_lane = CLS && tid < 16 && fx >= x0 && fx < x0 + kTX && (fx < g.lo[0] || fx > g.hi[0]) && y0 + (tid & 7) < op.ny;
+    // face pass: lanes 0..2 kTY-1 of wave 0 take the tile's x-face cells (row
+    // l % kTY, left face for l < kTY when the tile holds x = 0, right face otherwise)
+    const int fx = tid < kTY ? 0 : op.nx - 1;
+    const bool face_lane =
+        CLS && tid < 2 * kTY && fx >= x0 && fx < x0 + kTX && (fx < g.lo[0] || fx > g.hi[0]) && y0 + (tid % kTY) < op.ny;
     auto ycls = [&](int gy) { return gy == 0 ? 0 : (gy == op.ny - 1 ? 2 : 1); };
     // epilogue inputs of plane z (e.g. b), loaded a plane ahead
     using Pre = typename Epi::Pre;
@@ -337,14 +360,16 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
         const int gy = y0 + ty + q * (kBlock / kTX);
         return (gx_in && gy < op.ny) ? gx + (int64_t)op.nx * gy + sz * z : 0;
     };
-    auto preload = [&](Pre(&pre)[kRPT], int z) {
+    auto preload = [&](Pre(&pre)[kRPT], int z, bool need) {
 #pragma unroll
-        for(int q = 0; q < kRPT; q++) pre[q] = epi.load(cell_of(q, z));
+        for(int q = 0; q < kRPT; q++) pre[q] = epi.load(need ? cell_of(q, z) : 0);
     };
-    auto face_pre = [&](int z) -> Pre { return epi.load(face_lane ? fx + (int64_t)op.nx * (y0 + (tid & 7)) + sz * z : 0); };
+    auto face_pre = [&](int z, bool need) -> Pre {
+        return epi.load((need && face_lane) ? fx + (int64_t)op.nx * (y0 + (tid % kTY)) + sz * z : 0);
+    };
     // plane z from LDS (planes z-1, z, z+1 and, when needed, its class rows staged)
     auto compute = [&](int z, const Pre(&pre)[kRPT], Pre fpre) {
-        const TV *const P3[3] = {pl + ((z + 2) % 3) * kPL, pl + (z % 3) * kPL, pl + ((z + 1) % 3) * kPL};
+        const TV *const P3[3] = {pl + ((z + 2) % 3) * kPLS, pl + (z % 3) * kPLS, pl + ((z + 1) % 3) * kPLS};
         TC crow[PAT];
         bool zf = true;
         if constexpr(CLS) {
@@ -401,7 +426,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
         }
         if constexpr(CLS) {
             if(face_lane) {
-                const int rr = tid & 7, gy = y0 + rr;
+                const int rr = tid % kTY, gy = y0 + rr;
                 const int64_t c = fx + (int64_t)op.nx * gy + sz * z;
                 const int base = (rr + 1) * kW + (fx - x0) + 1;
                 const TC *row = crows + (ycls(gy) * 3 + (fx == 0 ? 0 : 2)) * 27;
@@ -411,42 +436,40 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
             }
         }
     };
-    fetch(ra, z0 - 1);
+    fetch(ra, z0 - 1, true);
     put(ra, (z0 + 2) % 3);
-    fetch(ra, z0);
+    fetch(ra, z0, true);
     put(ra, z0 % 3);
-    fetch(ra, z0 + 1);
-    if(z0 + 2 <= z1) fetch(rb, z0 + 2);
-    fetch_rows(z0);
-    preload(pa, z0);
-    Pre fa = face_pre(z0), fb{};
-    if(z0 + 1 < z1) {
-        preload(pb, z0 + 1);
-        fb = face_pre(z0 + 1);
-    }
+    fetch(ra, z0 + 1, true);
+    fetch(rb, z0 + 2, z0 + 2 <= z1);
+    fetch_rows(z0, true);
+    preload(pa, z0, true);
+    Pre fa = face_pre(z0, true);
+    preload(pb, z0 + 1, z0 + 1 < z1);
+    Pre fb = face_pre(z0 + 1, z0 + 1 < z1);
+    // Each half: the plane and class rows fetched earlier go into LDS (the
+    // class rows unconditionally, so that their load is waited for on every
+    // path), then the class rows and the plane three ahead are fetched, in
+    // that order.
     for(int z = z0; z < z1; z += 2) {
         put(ra, (z + 1) % 3);  // plane z+1
-        if(need_rows(z) && tid < 9 * 27) crows[tid] = rrow;
-        if(z + 3 <= z1) fetch(ra, z + 3);
-        if(z + 1 < z1) fetch_rows(z + 1);
+        if constexpr(CLS) crows[tid] = rrow;
+        fetch_rows(z + 1, z + 1 < z1);
+        fetch(ra, z + 3, z + 3 <= z1);
         __syncthreads();
         compute(z, pa, fa);
-        if(z + 2 < z1) {
-            preload(pa, z + 2);
-            fa = face_pre(z + 2);
-        }
+        preload(pa, z + 2, z + 2 < z1);
+        fa = face_pre(z + 2, z + 2 < z1);
         __syncthreads();
         if(z + 1 >= z1) break;
         put(rb, (z + 2) % 3);  // plane z+2
-        if(need_rows(z + 1) && tid < 9 * 27) crows[tid] = rrow;
-        if(z + 4 <= z1) fetch(rb, z + 4);
-        if(z + 2 < z1) fetch_rows(z + 2);
+        if constexpr(CLS) crows[tid] = rrow;
+        fetch_rows(z + 2, z + 2 < z1);
+        fetch(rb, z + 4, z + 4 <= z1);
         __syncthreads();
         compute(z + 1, pb, fb);
-        if(z + 3 < z1) {
-            preload(pb, z + 3);
-            fb = face_pre(z + 3);
-        }
+        preload(pb, z + 3, z + 3 < z1);
+        fb = face_pre(z + 3, z + 3 < z1);
         __syncthreads();
     }
 }
