@@ -304,8 +304,13 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
     // wave wait for the load) sits between a fetch and its put.
     struct Fetched {
         TV a[kNLD], b[FUSE ? kNLD : 1];
+        float c[Epi::kCorr ? kNLD : 1];
         bool zin;
     };
+    // correcting residual: x + d is staged (and written to p_out) unless the
+    // outer solve had already converged, when d is stale and x is copied
+    bool apply = false;
+    if constexpr(Epi::kCorr) apply = !epi.so->done;
     Fetched ra, rb;
     // Every load of the plane loop is issued unconditionally (a plane that is
     // not needed reads element 0 of the tile's first row), so the number of
@@ -322,6 +327,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
             const int64_t c = need ? zo + goff[j] : 0;  // always a valid address
             f.a[j] = x[c];
             if constexpr(FUSE) f.b[j] = rz[c];
+            if constexpr(Epi::kCorr) f.c[j] = epi.d[c];
         }
         __builtin_amdgcn_sched_barrier(0);
     };
@@ -330,6 +336,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
         for(int j = 0; j < kNLD; j++) {
             TV val;
             if constexpr(FUSE) val = f.b[j] + beta * f.a[j];
+            else if constexpr(Epi::kCorr) val = f.a[j] + (apply ? (double)f.c[j] : 0.0);
             else val = f.a[j];
             pl[slot * kPLS + tid + j * kBlock] = (f.zin && gok[j]) ? val : TV(0);  // lanes past kPL fill padding
         }
@@ -421,7 +428,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
                 }
             }
             const TV xc = P3[1][base];
-            if constexpr(FUSE) p_out[c] = xc;
+            if constexpr(FUSE || Epi::kCorr) p_out[c] = xc;
             epi(c, acc, xc, dg, pre[q], v);
         }
         if constexpr(CLS) {
@@ -431,7 +438,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
                 const int base = (rr + 1) * kW + (fx - x0) + 1;
                 const TC *row = crows + (ycls(gy) * 3 + (fx == 0 ? 0 : 2)) * 27;
                 const TV xc = P3[1][base];
-                if constexpr(FUSE) p_out[c] = xc;
+                if constexpr(FUSE || Epi::kCorr) p_out[c] = xc;
                 epi(c, row27_lds(row, P3, base), xc, (double)row[13], fpre, v);
             }
         }
@@ -525,7 +532,9 @@ struct NoPre {};
 
 template <typename TV> struct EpiSpmv {
     static constexpr int NR = 1;
-    static constexpr bool kDiag = false;
+    static constexpr bool kDiag = false, kCorr = false;
+    static constexpr const float *d = nullptr;
+    static constexpr const CgScalars *so = nullptr;
     using Pre = NoPre;
     TV *q;
     __device__ Pre load(int64_t) const { return {}; }
@@ -538,7 +547,9 @@ template <typename TV> struct EpiSpmv {
 // r = b - A x, p = r (or z = D^-1 r, p = z); partial r.r, r.z, b.b
 template <bool JACOBI> struct EpiRes64 {
     static constexpr int NR = 3;
-    static constexpr bool kDiag = JACOBI;
+    static constexpr bool kDiag = JACOBI, kCorr = false;
+    static constexpr const float *d = nullptr;
+    static constexpr const CgScalars *so = nullptr;
     using Pre = double;
     const double *b;
     double *r, *p, *z;
@@ -561,23 +572,28 @@ template <bool JACOBI> struct EpiRes64 {
 };
 
 // mixed-precision outer residual r = b - A x in FP64, handed to the FP32 solve
-struct EpiResOuter {
+// (whose first direction is r itself and whose correction starts from zero,
+// so neither is written here).  CORR: x is first corrected by the previous
+// sweep's FP32 correction d, x + d going to the kernel's p_out (a second FP64
+// buffer: other blocks still read x as their halo).
+template <bool CORR> struct EpiResOuterT {
     static constexpr int NR = 2;  // r.r, b.b
-    static constexpr bool kDiag = false;
+    static constexpr bool kDiag = false, kCorr = CORR;
     using Pre = double;
     const double *b;
-    float *r32, *p32, *d32;
+    float *r32;
+    const float *d;        // CORR: the correction
+    const CgScalars *so;   // CORR: outer solve state (done: d is stale)
     __device__ Pre load(int64_t c) const { return b[c]; }
     __device__ void operator()(int64_t c, double ax, double, double, Pre bc, double (&v)[NR]) const {
         const double rc = bc - ax;
-        const float rf = (float)rc;
-        r32[c] = rf;
-        p32[c] = rf;
-        d32[c] = 0.0f;
+        r32[c] = (float)rc;
         v[0] += rc * rc;
         v[1] += bc * bc;
     }
 };
+using EpiResOuter = EpiResOuterT<false>;
+using EpiResOuterCorr = EpiResOuterT<true>;
 
 // Elementwise kernels keep four cells in flight per thread (all loads issued
 // before any use).
@@ -587,7 +603,7 @@ constexpr int kIlp = 4;
 template <typename TV, typename TX, typename TC>
 __global__ __launch_bounds__(kBlock) void k_update(OpView<TC> op, TX *__restrict__ x, const TV *__restrict__ p, const TV *__restrict__ q,
                                                    TV *__restrict__ r, TV *__restrict__ z, int jacobi, int k, const double *Pin,
-                                                   int nprev, double *Pout, CgScalars *sc) {
+                                                   int nprev, double *Pout, CgScalars *sc, int fresh) {
     if(sc->done) return;
     double pq[1];
     sum_partials<1>(Pin, nprev, pq);
@@ -597,6 +613,9 @@ __global__ __launch_bounds__(kBlock) void k_update(OpView<TC> op, TX *__restrict
         sc->alpha = alpha;
     }
     const TV a = (TV)alpha;
+    // fresh: x starts from zero (the mixed solve's correction), so the first
+    // iteration writes a p without reading x
+    const bool x0 = fresh && k == 0;
     int64_t beg, end;
     chunk_range(op.n, beg, end);
     double v[2] = {0.0, 0.0};
@@ -608,7 +627,7 @@ __global__ __launch_bounds__(kBlock) void k_update(OpView<TC> op, TX *__restrict
         for(int u = 0; u < kIlp; u++) {
             const int64_t c = c0 + u * kBlock;
             cc[u] = c < end ? c : end - 1;
-            xv[u] = x[cc[u]];
+            xv[u] = x0 ? TX(0) : x[cc[u]];
             pv[u] = p[cc[u]];
             qv[u] = q[cc[u]];
             rv[u] = r[cc[u]];
@@ -637,30 +656,6 @@ __global__ __launch_bounds__(kBlock) void k_update(OpView<TC> op, TX *__restrict
 // solve had already converged (its inner and correction kernels were empty)
 __global__ void k_note_sweep(const CgScalars *si, const CgScalars *so, int *out) {
     if(threadIdx.x == 0) out[0] = so->done ? -1 : si->iters - 1;
-}
-
-__global__ __launch_bounds__(kBlock) void k_correct(int64_t n, double *__restrict__ x, const float *__restrict__ d32,
-                                                    const CgScalars *so) {
-    if(so->done) return;
-    int64_t beg, end;
-    chunk_range(n, beg, end);
-    for(int64_t c0 = beg + threadIdx.x; c0 < end; c0 += kIlp * kBlock) {
-        int64_t cc[kIlp];
-        double xv[kIlp];
-        float dv[kIlp];
-#pragma unroll
-        for(int u = 0; u < kIlp; u++) {
-            const int64_t c = c0 + u * kBlock;
-            cc[u] = c < end ? c : end - 1;
-            xv[u] = x[cc[u]];
-            dv[u] = d32[cc[u]];
-        }
-#pragma unroll
-        for(int u = 0; u < kIlp; u++) {
-            const int64_t c = c0 + u * kBlock;
-            if(c < end) x[c] = xv[u] + (double)dv[u];
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -950,7 +945,7 @@ int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x
             });
             if(nst < 0) return -1;
             // alpha from P1; x += alpha p; r -= alpha q; r.r, r.z -> P0
-            k_update<double, double, double><<<grid, kBlock, 0, s>>>(op, x, pout, q, r, z, cfg.jacobi, k, P1, nst, P0, sc);
+            k_update<double, double, double><<<grid, kBlock, 0, s>>>(op, x, pout, q, r, z, cfg.jacobi, k, P1, nst, P0, sc, 0);
         }
         MONO_CHECK(hipGetLastError());
         enq += chunk;
@@ -976,18 +971,30 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
     int inner_total = 0, corrections = 0;
     int chunk0 = last_inner > 2 ? last_inner + 1 : 4;
     CgScalars *h_out = host_sc, *h_in = host_sc + 1;
+    // The correction x += d of a sweep is applied by the next sweep's
+    // residual, which reads x from one FP64 buffer and writes x + d to the
+    // other (x and the workspace's FP64 vector r, unused by this solve).
+    double *xbuf[2] = {x, r};
+    int cur = 0;
     auto residual = [&](int sweep, const CgSetup &cs) -> int {
-        // outer residual in FP64 -> r32 = p32 = r, d32 = 0; r.r, b.b -> P0
+        // outer residual in FP64 -> r32; r.r, b.b -> P0
         int nres = 0;
         with_op(op64, [&](auto C_, auto P_) {
             constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
-            nres = launch_stencil<double, double, C, P, false>(op64, x, nullptr, nullptr, EpiResOuter{b, r32, pb[0], d32}, cs, -1, 0,
-                                                               nullptr, 0, P0, sc, nullptr, s);
+            if(sweep == 0)
+                nres = launch_stencil<double, double, C, P, false>(op64, xbuf[cur], nullptr, nullptr, EpiResOuter{b, r32, nullptr, nullptr},
+                                                                   cs, -1, 0, nullptr, 0, P0, sc, nullptr, s);
+            else
+                nres = launch_stencil<double, double, C, P, false>(op64, xbuf[cur], nullptr, xbuf[cur ^ 1],
+                                                                   EpiResOuterCorr{b, r32, d32, sc}, cs, -1, 0, nullptr, 0, P0, sc,
+                                                                   nullptr, s);
         });
+        if(sweep > 0) cur ^= 1;
         return nres;
     };
     auto inner = [&](const CgSetup &cs, int k, int nres) -> int {
-        float *pin = pb[k & 1], *pout = pb[(k + 1) & 1];
+        // the first direction is the residual itself
+        float *pin = k == 0 ? r32 : pb[k & 1], *pout = pb[(k + 1) & 1];
         int nst = 0;
         with_op(op32, [&](auto C_, auto P_) {
             constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
@@ -995,7 +1002,7 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
                                                            k == 0 ? nres : grid, P1, sc_in, sc, s);
         });
         if(nst < 0) return -1;
-        k_update<float, float, float><<<grid, kBlock, 0, s>>>(op32, d32, pout, q32, r32, nullptr, 0, k, P1, nst, P0, sc_in);
+        k_update<float, float, float><<<grid, kBlock, 0, s>>>(op32, d32, pout, q32, r32, nullptr, 0, k, P1, nst, P0, sc_in, 1);
         return 0;
     };
     // Sweeps enqueued without polling, as many as the previous solve needed,
@@ -1012,7 +1019,6 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
         for(int k = 0; k < last_sched[sweep] + 2; k++)
             if(inner(cs, k, nres)) return -1;
         k_note_sweep<<<1, 64, 0, s>>>(sc_in, sc, sweep_its + sweep);
-        k_correct<<<grid, kBlock, 0, s>>>(op64.n, x, d32, sc);
         MONO_CHECK(hipGetLastError());
     }
     std::vector<int> sched;
@@ -1051,10 +1057,10 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
         sched.push_back(h_in->iters - 1);
         corrections++;
         if(sweep == 0) last_inner = h_in->iters - 1;
-        k_correct<<<grid, kBlock, 0, s>>>(op64.n, x, d32, sc);
-        MONO_CHECK(hipGetLastError());
         chunk0 = 2;
     }
+    // the solution lives in the buffer the last residual wrote
+    if(cur != 0) MONO_CHECK(hipMemcpyAsync(x, xbuf[cur], sizeof(double) * op64.n, hipMemcpyDeviceToDevice, s));
     if(sched.size() > (size_t)kMaxBlindSweeps) sched.resize(kMaxBlindSweeps);
     last_sched = sched;
     if(st) {
