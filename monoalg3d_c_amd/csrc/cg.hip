@@ -447,9 +447,9 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
     put(ra, (z0 + 2) % 3);
     fetch(ra, z0, true);
     put(ra, z0 % 3);
+    fetch_rows(z0, true);
     fetch(ra, z0 + 1, true);
     fetch(rb, z0 + 2, z0 + 2 <= z1);
-    fetch_rows(z0, true);
     preload(pa, z0, true);
     Pre fa = face_pre(z0, true);
     preload(pb, z0 + 1, z0 + 1 < z1);
