@@ -506,6 +506,128 @@ __device__ __forceinline__ void residency_probe(int spin, double *out) {
     }
 }
 
+// One-plane domains (nz == 1) with 7-point rows (5 in the plane): one wave per
+// (64-column strip, run of kRows2d rows).  A lane owns one column; all loads of
+// the run (its rows and the one above and below, the strip's two edge columns,
+// the row codes and the epilogue inputs) are issued first, in straight-line
+// code, then the rows are evaluated in order, x neighbours by lane shifts.  No
+// LDS staging and no barriers.  The in-plane terms are summed in the order of
+// RowPattern<7> as the plane kernel does (its z terms are products with the
+// zero padding), so both kernels give the same rows bit for bit.
+constexpr int kRows2d = 8;
+
+template <typename TV> __device__ __forceinline__ TV lane_shift(TV v, int src) { return __shfl(v, src); }
+
+template <typename TV, typename TC, int CM, bool FUSE, class Epi>
+__device__ __forceinline__ void strip_run(const OpView<TC> &op, int strip, int chunk, const TV *__restrict__ x,
+                                          const TV *__restrict__ rz, TV beta, TV *__restrict__ p_out, const Epi &epi,
+                                          const TC *ctab, bool apply, double (&v)[Epi::NR]) {
+    constexpr int R = kRows2d, W = R + 2;
+    const int lane = threadIdx.x & 63;
+    const int x0 = strip * 64, gx = x0 + lane;
+    const bool xin = gx < op.nx;
+    const int ex = lane < 32 ? x0 - 1 : x0 + 64;  // the edge column this lane loads
+    const bool exin = ex >= 0 && ex < op.nx;
+    const int y0 = chunk * R;
+    // raw loads of rows y0-1 .. y0+R (window slot w holds row y0-1+w)
+    TV ca[W], ea[W], cb[FUSE ? W : 1], eb[FUSE ? W : 1];
+    float cd[Epi::kCorr ? W : 1], ed[Epi::kCorr ? W : 1];
+    bool yin[W];
+#pragma unroll
+    for(int w = 0; w < W; w++) {
+        const int y = y0 - 1 + w;
+        yin[w] = y >= 0 && y < op.ny;
+        const int64_t row = (int64_t)(yin[w] ? y : 0) * op.nx;
+        const int64_t c = (yin[w] && xin) ? row + gx : 0, ce = (yin[w] && exin) ? row + ex : 0;
+        ca[w] = x[c];
+        ea[w] = x[ce];
+        if constexpr(FUSE) {
+            cb[w] = rz[c];
+            eb[w] = rz[ce];
+        }
+        if constexpr(Epi::kCorr) {
+            cd[w] = epi.d[c];
+            ed[w] = epi.d[ce];
+        }
+    }
+    using Pre = typename Epi::Pre;
+    Pre pre[R];
+    uint8_t code[CM == 2 ? R : 1];
+#pragma unroll
+    for(int r = 0; r < R; r++) {
+        const int y = y0 + r;
+        const int64_t c = (xin && y < op.ny) ? gx + (int64_t)op.nx * y : 0;
+        pre[r] = epi.load(c);
+        if constexpr(CM == 2) code[r] = op.codes[c];
+    }
+    auto val = [&](TV a, TV b, float d, bool ok) -> TV {
+        TV t;
+        if constexpr(FUSE) t = b + beta * a;
+        else if constexpr(Epi::kCorr) t = a + (apply ? (double)d : 0.0);
+        else t = a;
+        return ok ? t : TV(0);
+    };
+    // combined values of the window (centre column and edge column)
+    TV cv[W], ev[W];
+#pragma unroll
+    for(int w = 0; w < W; w++) {
+        cv[w] = val(ca[w], FUSE ? cb[FUSE ? w : 0] : TV(0), Epi::kCorr ? cd[Epi::kCorr ? w : 0] : 0.0f, yin[w] && xin);
+        ev[w] = val(ea[w], FUSE ? eb[FUSE ? w : 0] : TV(0), Epi::kCorr ? ed[Epi::kCorr ? w : 0] : 0.0f, yin[w] && exin);
+    }
+#pragma unroll
+    for(int r = 0; r < R; r++) {
+        const int y = y0 + r, w = r + 1;
+        const TV left = lane_shift(cv[w], lane - 1), right = lane_shift(cv[w], lane + 1);
+        // lane 0's left neighbour and lane 63's right neighbour are the edge
+        // columns, loaded by lanes 0..31 and 32..63 respectively
+        const TV xl = lane == 0 ? ev[w] : left;
+        const TV xr = lane == 63 ? ev[w] : right;
+        const TV nb[5] = {cv[w], xr, xl, cv[w + 1], cv[w - 1]};  // RowPattern<7> slots 0..4
+        if(!xin || y >= op.ny) continue;
+        const int64_t c = gx + (int64_t)op.nx * y;
+        TV acc = TV(0);
+        double dg = 0.0;
+        if constexpr(CM == 0) {
+            const uint32_t cu = (uint32_t)c;
+#pragma unroll
+            for(int k = 0; k < 5; k++) acc += (TV)(op.percell + (int64_t)k * op.n)[cu] * nb[k];
+            if constexpr(Epi::kDiag) dg = (double)op.percell[cu];
+        } else {
+            const TC *rowc = ctab + (int)code[r] * 7;
+#pragma unroll
+            for(int k = 0; k < 5; k++) acc += (TV)rowc[k] * nb[k];
+            if constexpr(Epi::kDiag) dg = (double)rowc[0];
+        }
+        if constexpr(FUSE || Epi::kCorr) p_out[c] = cv[w];
+        epi(c, acc, cv[w], dg, pre[r], v);
+    }
+}
+
+template <typename TV, typename TC, int CM, bool FUSE, class Epi>
+__global__ __launch_bounds__(kBlock) void k_stencil2d(OpView<TC> op, int nitems, const TV *__restrict__ x, const TV *__restrict__ rz,
+                                                      TV *__restrict__ p_out, Epi epi, CgSetup cs, int mode, int k,
+                                                      const double *Pin, int nprev, double *Pout, CgScalars *sc, CgScalars *so) {
+    double betad = 0.0;
+    if(mode >= 0 && !iter_prologue(mode, k, Pin, nprev, sc, so, cs, betad)) return;
+    const TV beta = (TV)betad;
+    __shared__ TC ctab[CM == 2 ? kCodes * 7 : 1];
+    if constexpr(CM == 2) {
+        for(int i = threadIdx.x; i < kCodes * 7; i += kBlock) ctab[i] = op.ctab[i];
+        __syncthreads();
+    }
+    bool apply = false;
+    if constexpr(Epi::kCorr) apply = !epi.so->done;
+    double v[Epi::NR];
+#pragma unroll
+    for(int j = 0; j < Epi::NR; j++) v[j] = 0.0;
+    const int nstrips = (op.nx + 63) / 64;
+    // items (chunk, strip), strip fastest; each XCD takes a contiguous range
+    const int waves = gridDim.x * (kBlock / 64);
+    for(int item = xcd_chunk(blockIdx.x, gridDim.x) * (kBlock / 64) + (int)(threadIdx.x >> 6); item < nitems; item += waves)
+        strip_run<TV, TC, CM, FUSE, Epi>(op, item % nstrips, item / nstrips, x, rz, beta, p_out, epi, ctab, apply, v);
+    write_partials<Epi::NR>(v, Pout);
+}
+
 // Residual kernels: no prologue.  Iteration kernel (q = A p): prologue first.
 template <typename TV, typename TC, int CM, int PAT, bool FUSE, class Epi>
 __global__ __launch_bounds__(kBlock) void k_stencil(OpView<TC> op, TileGeom g, const TV *__restrict__ x, const TV *__restrict__ rz,
@@ -883,6 +1005,16 @@ static int poll(CgScalars *dev, CgScalars *host, hipStream_t s) {
 template <typename TV, typename TC, int C, int P, bool FUSE, class Epi>
 static int launch_stencil(const OpView<TC> &op, const TV *x, const TV *rz, TV *p_out, const Epi &epi, const CgSetup &cs, int mode,
                           int k, const double *Pin, int nprev, double *Pout, CgScalars *sc, CgScalars *so, hipStream_t s) {
+    if constexpr(P == 7 && (C == 0 || C == 2)) {
+        if(op.nz == 1) {  // one plane: the strip kernel
+            const int nitems = ((op.nx + 63) / 64) * ((op.ny + kRows2d - 1) / kRows2d);
+            int nb = (nitems + kBlock / 64 - 1) / (kBlock / 64);
+            if(nb > kMaxBlocks) nb = kMaxBlocks;  // the waves then loop over the items
+            if(nb >= 16) nb = nb / 8 * 8;
+            k_stencil2d<TV, TC, C, FUSE, Epi><<<nb, kBlock, 0, s>>>(op, nitems, x, rz, p_out, epi, cs, mode, k, Pin, nprev, Pout, sc, so);
+            return nb;
+        }
+    }
     constexpr auto K = &k_stencil<TV, TC, C, P, FUSE, Epi>;
     static int slots[64] = {};
     int dev = 0;
