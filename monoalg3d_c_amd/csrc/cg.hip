@@ -514,7 +514,10 @@ __device__ __forceinline__ void residency_probe(int spin, double *out) {
 // LDS staging and no barriers.  The in-plane terms are summed in the order of
 // RowPattern<7> as the plane kernel does (its z terms are products with the
 // zero padding), so both kernels give the same rows bit for bit.
-constexpr int kRows2d = 8;
+#ifndef MONO_ROWS_2D
+#define MONO_ROWS_2D 8
+#endif
+constexpr int kRows2d = MONO_ROWS_2D;
 
 template <typename TV> __device__ __forceinline__ TV lane_shift(TV v, int src) { return __shfl(v, src); }
 
