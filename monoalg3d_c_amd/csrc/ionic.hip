@@ -63,7 +63,22 @@ template <typename T> __device__ __forceinline__ double cell_stim(const IonicLau
     return st;
 }
 
-template <int MODEL, typename T>
+// Writes each new ORd state to its HBM row as soon as the model has it final.
+template <typename T> struct RowStore {
+    T *sv;
+    size_t pitch;
+    int64_t i;
+    __device__ __forceinline__ void operator()(int s, T val) const { sv[(size_t)s * pitch + i] = val; }
+};
+
+// ONE: the launch advances one ODE step per PDE step (num_steps == 1, every
+// configuration with dt_ode = dt_pde).  Without the loop-carried state array
+// the compiler keeps far fewer registers live, and ORd stores each gate right
+// after its update: the FP64 ORd kernel drops from 256 VGPRs with 236 B/lane
+// of scratch spills to ~180 VGPRs without (4.1 -> 2.6 ms at 16M cells).  The
+// FP32 kernels are HBM-bound and keep the loop form: the one-step FP32 ORd
+// kernel runs at 6 waves/SIMD and measured 4% slower than the two-wave one.
+template <int MODEL, typename T, bool ONE>
 __device__ __forceinline__ void ionic_step_body(const IonicLaunch<T> &a) {
     constexpr int NEQ = ModelTraits<MODEL>::neq;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -76,12 +91,19 @@ __device__ __forceinline__ void ionic_step_body(const IonicLaunch<T> &a) {
     double V = a.first_step ? (double)a.sv[i] : a.v[i];
 
     const double st = cell_stim(a, i);
+    if constexpr(ONE && MODEL == kORd2011) {
+        y[0] = T(V);
+        V = V + (double)ord2011_step<T>(y, T(st), a.dt, RowStore<T>{a.sv, a.pitch, i});
+        if(a.write_vm_row) a.sv[i] = T(V);
+        if(a.b) a.b[i] = V * a.alpha;
+        return;
+    }
     float scale[4] = {1.f, 1.f, 1.f, 1.f};
     if(a.scale) {
 #pragma unroll
         for(int k = 0; k < 4; k++) scale[k] = a.scale[4 * i + k];
     }
-    for(int k = 0; k < a.num_steps; k++) {
+    for(int k = 0; k < (ONE ? 1 : a.num_steps); k++) {
         y[0] = T(V);
         const T dv = model_step<MODEL, T>(y, T(st), a.dt, scale);
         V = V + (double)dv;
@@ -185,15 +207,16 @@ __global__ __launch_bounds__(256) void k_ionic_adaptive(IonicLaunch<T> a) {
 
 // FP32: the register count the compiler picks.  FP64 models are bound by
 // their transcendental and division arithmetic; at two waves per SIMD the
-// state loads of one wave overlap the arithmetic of the other (a few
-// registers spill to scratch, which measures faster than one wave per SIMD).
+// state loads of one wave overlap the arithmetic of the other (the multi-step
+// kernels spill a few registers to scratch, which measures faster than one
+// wave per SIMD).
 template <int MODEL>
 __global__ __launch_bounds__(256) void k_ionic_step32(IonicLaunch<float> a) {
-    ionic_step_body<MODEL, float>(a);
+    ionic_step_body<MODEL, float, false>(a);
 }
-template <int MODEL>
+template <int MODEL, bool ONE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_ionic_step64(IonicLaunch<double> a) {
-    ionic_step_body<MODEL, double>(a);
+    ionic_step_body<MODEL, double, ONE>(a);
 }
 
 template <int MODEL, typename T>
@@ -242,20 +265,28 @@ double model_initial_v(int model) {
     return 0.00000820413566106744;
 }
 
-template <typename T>
-static hipError_t launch_step_t(int model, const IonicLaunch<T> &a, hipStream_t s) {
+template <bool ONE>
+static hipError_t launch_step64(int model, const IonicLaunch<double> &a, hipStream_t s) {
     const int block = 256;
     const unsigned grid = (unsigned)((a.n + block - 1) / block);
+    switch(model) {
+    case kTT2006: k_ionic_step64<kTT2006, ONE><<<grid, block, 0, s>>>(a); break;
+    case kORd2011: k_ionic_step64<kORd2011, ONE><<<grid, block, 0, s>>>(a); break;
+    case kMS2003: k_ionic_step64<kMS2003, ONE><<<grid, block, 0, s>>>(a); break;
+    case kCRN1998: k_ionic_step64<kCRN1998, ONE><<<grid, block, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_step_t(int model, const IonicLaunch<T> &a, hipStream_t s) {
     if(a.n == 0) return hipSuccess;
     if constexpr(sizeof(T) == 8) {
-        switch(model) {
-        case kTT2006: k_ionic_step64<kTT2006><<<grid, block, 0, s>>>(a); break;
-        case kORd2011: k_ionic_step64<kORd2011><<<grid, block, 0, s>>>(a); break;
-        case kMS2003: k_ionic_step64<kMS2003><<<grid, block, 0, s>>>(a); break;
-        case kCRN1998: k_ionic_step64<kCRN1998><<<grid, block, 0, s>>>(a); break;
-        default: return hipErrorInvalidValue;
-        }
+        return a.num_steps == 1 ? launch_step64<true>(model, a, s) : launch_step64<false>(model, a, s);
     } else {
+        const int block = 256;
+        const unsigned grid = (unsigned)((a.n + block - 1) / block);
         switch(model) {
         case kTT2006: k_ionic_step32<kTT2006><<<grid, block, 0, s>>>(a); break;
         case kORd2011: k_ionic_step32<kORd2011><<<grid, block, 0, s>>>(a); break;
@@ -263,8 +294,8 @@ static hipError_t launch_step_t(int model, const IonicLaunch<T> &a, hipStream_t 
         case kCRN1998: k_ionic_step32<kCRN1998><<<grid, block, 0, s>>>(a); break;
         default: return hipErrorInvalidValue;
         }
+        return hipGetLastError();
     }
-    return hipGetLastError();
 }
 
 template <typename T>

@@ -268,15 +268,24 @@ namespace mono {
 
 template <typename T> __device__ __forceinline__ T cube(T x) { return x * x * x; }
 
+// State sink of a model step: st(s, value) is called once per state s >= 1 as
+// soon as its new value is final.  A kernel that passes a sink writing state
+// rows straight to HBM stores each gate right after its update, so the gate is
+// dead for the rest of the step instead of held in registers until the end.
+struct NullStore {
+    template <typename T> __device__ __forceinline__ void operator()(int, T) const {}
+};
+
 // ---------------------------------------------------------------------------
 // O'Hara-Rudy 2011, endocardial cell (Ohara_Rudy_2011.c:110-691, celltype 0).
 // In-place and in statement order: the gates advance by Rush-Larsen from V_n,
 // the currents use the updated gates, V by forward Euler (:570), then CaMK,
 // release fluxes and concentrations.  y[0] (V) is read-only; returns dV.
 // pow(x, n) with integer n is evaluated by products (<= 2 ulp from pow).
+// The new states are written to y[1..40] and also passed to `st`.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T ord2011_step(T *y, T stim, T dt) {
+template <typename T, typename St = NullStore>
+__device__ __forceinline__ T ord2011_step(T *y, T stim, T dt, St st = St()) {
     const T v = y[0];
     T nai = y[1], nass = y[2], ki = y[3], kss = y[4], cai = y[5], cass = y[6];
     T cansr = y[7], cajsr = y[8], m = y[9], hf = y[10], hs = y[11], j = y[12], hsp = y[13];
@@ -317,35 +326,44 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt) {
     const T mss = T(1.0f) / (T(1.0f) + dexp<T>((-(v + T(39.57f))) / T(9.871f)));
     const T tm = T(1.0f) / (T(6.765f) * dexp<T>((v + T(11.64f)) / T(34.77f)) + T(8.552f) * dexp<T>(-(v + T(77.42f)) / T(5.955f)));
     m = rl_from<T>(mss, m, dt, tm);
+    st(9, m);
     const T hss = T(1.0f) / (T(1) + dexp<T>((v + T(82.90f)) / T(6.086f)));
     const T thf = T(1.0f) / (T(1.432e-5f) * dexp<T>(-(v + T(1.196f)) / T(6.285f)) + T(6.149f) * dexp<T>((v + T(0.5096f)) / T(20.27f)));
     const T ths = T(1.0f) / (T(0.009794f) * dexp<T>(-(v + T(17.95f)) / T(28.05f)) + T(0.3343f) * dexp<T>((v + T(5.730f)) / T(56.66f)));
     const T Ahf = T(0.99);
     const T Ahs = T(1.0f) - Ahf;
     hf = rl_from<T>(hss, hf, dt, thf);
+    st(10, hf);
     hs = rl_from<T>(hss, hs, dt, ths);
+    st(11, hs);
     const T h = Ahf * hf + Ahs * hs;
     const T jss = hss;
     const T tj = T(2.038f) + T(1.0f) / (T(0.02136f) * dexp<T>(-(v + T(100.6f)) / T(8.281f)) + T(0.3052f) * dexp<T>((v + T(0.9941f)) / T(38.45f)));
     j = rl_from<T>(jss, j, dt, tj);
+    st(12, j);
     const T hssp = T(1.0f) / (T(1.0f) + dexp<T>((v + T(89.1f)) / T(6.086f)));
     const T thsp = T(3.0f) * ths;
     hsp = rl_from<T>(hssp, hsp, dt, thsp);
+    st(13, hsp);
     const T hp = Ahf * hf + Ahs * hsp;
     const T tjp = T(1.46f) * tj;
     jp = rl_from<T>(jss, jp, dt, tjp);
+    st(14, jp);
     const T GNa = T(75);
     const T INa = GNa * (v - ENa) * m * m * m * ((T(1.0f) - fCaMK) * h * j + fCaMK * hp * jp);
 
     // INaL
     const T mLss = T(1.0f) / (T(1.0f) + dexp<T>((-(v + T(42.85f))) / T(5.264f)));
     mL = rl_from<T>(mLss, mL, dt, tm);
+    st(15, mL);
     const T hLss = T(1.0f) / (T(1.0f) + dexp<T>((v + T(87.61f)) / T(7.488f)));
     const T thL = T(200.0);
     hL = rl_from<T>(hLss, hL, dt, thL);
+    st(16, hL);
     const T hLssp = T(1.0f) / (T(1.0f) + dexp<T>((v + T(93.81f)) / T(7.488f)));
     const T thLp = T(3.0f) * thL;
     hLp = rl_from<T>(hLssp, hLp, dt, thLp);
+    st(17, hLp);
     const T GNaL = T(0.0075);
     const T INaL = GNaL * (v - ENa) * mL * ((T(1.0f) - fCaMK) * hL + fCaMK * hLp);
 
@@ -354,22 +372,28 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt) {
     const T ta = T(1.0515f) / (T(1.0f) / (T(1.2089f) * (T(1.0f) + dexp<T>(-(v - T(18.4099f)) / T(29.3814f)))) +
                                T(3.5f) / (T(1.0f) + dexp<T>((v + T(100.0f)) / T(29.3814f))));
     a = rl_from<T>(ass, a, dt, ta);
+    st(18, a);
     const T iss = T(1.0f) / (T(1.0f) + dexp<T>((v + T(43.94f)) / T(5.711f)));
     const T tiF = T(4.562f) + T(1.0f) / (T(0.3933f) * dexp<T>((-(v + T(100.0f))) / T(100.0f)) + T(0.08004f) * dexp<T>((v + T(50.0f)) / T(16.59f)));
     const T tiS = T(23.62f) + T(1.0f) / (T(0.001416f) * dexp<T>((-(v + T(96.52f))) / T(59.05f)) + T(1.780e-8f) * dexp<T>((v + T(114.1f)) / T(8.079f)));
     const T AiF = T(1.0f) / (T(1.0f) + dexp<T>((v - T(213.6f)) / T(151.2f)));
     const T AiS = T(1.0f) - AiF;
     iF = rl_from<T>(iss, iF, dt, tiF);
+    st(19, iF);
     iS = rl_from<T>(iss, iS, dt, tiS);
+    st(20, iS);
     const T ii = AiF * iF + AiS * iS;
     const T assp = T(1.0f) / (T(1.0f) + dexp<T>((-(v - T(24.34f))) / T(14.82f)));
     ap = rl_from<T>(assp, ap, dt, ta);
+    st(21, ap);
     const T dti_develop = T(1.354f) + T(1.0e-4f) / (dexp<T>((v - T(167.4f)) / T(15.89f)) + dexp<T>(-(v - T(12.23f)) / T(0.2154f)));
     const T dti_recover = T(1.0f) - T(0.5f) / (T(1.0f) + dexp<T>((v + T(70.0f)) / T(20.0f)));
     const T tiFp = dti_develop * dti_recover * tiF;
     const T tiSp = dti_develop * dti_recover * tiS;
     iFp = rl_from<T>(iss, iFp, dt, tiFp);
+    st(22, iFp);
     iSp = rl_from<T>(iss, iSp, dt, tiSp);
+    st(23, iSp);
     const T ip = AiF * iFp + AiS * iSp;
     const T Gto = T(0.02);
     const T Ito = Gto * (v - EK) * ((T(1.0f) - fCaMK) * a * ii + fCaMK * ap * ip);
@@ -378,13 +402,16 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt) {
     const T dss = T(1.0f) / (T(1.0f) + dexp<T>((-(v + T(3.940f))) / T(4.230f)));
     const T td = T(0.6f) + T(1.0f) / (dexp<T>(T(-0.05f) * (v + T(6.0f))) + dexp<T>(T(0.09f) * (v + T(14.0f))));
     d = rl_from<T>(dss, d, dt, td);
+    st(24, d);
     const T fss = T(1.0f) / (T(1.0f) + dexp<T>((v + T(19.58f)) / T(3.696f)));
     const T tff = T(7.0f) + T(1.0f) / (T(0.0045f) * dexp<T>(-(v + T(20.0f)) / T(10.0f)) + T(0.0045f) * dexp<T>((v + T(20.0f)) / T(10.0f)));
     const T tfs = T(1000.0f) + T(1.0f) / (T(0.000035f) * dexp<T>(-(v + T(5.0f)) / T(4.0f)) + T(0.000035f) * dexp<T>((v + T(5.0f)) / T(6.0f)));
     const T Aff = T(0.6);
     const T Afs = T(1.0f) - Aff;
     ff = rl_from<T>(fss, ff, dt, tff);
+    st(25, ff);
     fs = rl_from<T>(fss, fs, dt, tfs);
+    st(26, fs);
     const T f = Aff * ff + Afs * fs;
     const T fcass = fss;
     const T tfcaf = T(7.0f) + T(1.0f) / (T(0.04f) * dexp<T>(-(v - T(4.0f)) / T(7.0f)) + T(0.04f) * dexp<T>((v - T(4.0f)) / T(7.0f)));
@@ -392,15 +419,20 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt) {
     const T Afcaf = T(0.3f) + T(0.6f) / (T(1.0f) + dexp<T>((v - T(10.0f)) / T(10.0f)));
     const T Afcas = T(1.0f) - Afcaf;
     fcaf = rl_from<T>(fcass, fcaf, dt, tfcaf);
+    st(27, fcaf);
     fcas = rl_from<T>(fcass, fcas, dt, tfcas);
+    st(28, fcas);
     const T fca = Afcaf * fcaf + Afcas * fcas;
     const T tjca = T(75.0);
     jca = rl_from<T>(fcass, jca, dt, tjca);
+    st(29, jca);
     const T tffp = T(2.5f) * tff;
     ffp = rl_from<T>(fss, ffp, dt, tffp);
+    st(31, ffp);
     const T fp = Aff * ffp + Afs * fs;
     const T tfcafp = T(2.5f) * tfcaf;
     fcafp = rl_from<T>(fcass, fcafp, dt, tfcafp);
+    st(32, fcafp);
     const T fcap = Afcaf * fcafp + Afcas * fcas;
     const T Kmn = T(0.002);
     const T k2n = T(1000.0);
@@ -409,6 +441,7 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt) {
     const T kq2 = kq * kq;
     const T anca = T(1.0f) / (k2n / km2n + kq2 * kq2);
     nca = anca * k2n / km2n - (anca * k2n / km2n - nca) * dexp<T>(-km2n * dt);
+    st(30, nca);
     const T e2 = dexp<T>(T(2.0f) * vfrt);
     const T e1 = dexp<T>(T(1.0f) * vfrt);
     T PhiCaL, PhiCaNa, PhiCaK;
@@ -443,7 +476,9 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt) {
     const T Axrf = T(1.0f) / (T(1.0f) + dexp<T>((v + T(54.81f)) / T(38.21f)));
     const T Axrs = T(1.0f) - Axrf;
     xrf = rl_from<T>(xrss, xrf, dt, txrf);
+    st(33, xrf);
     xrs = rl_from<T>(xrss, xrs, dt, txrs);
+    st(34, xrs);
     const T xr = Axrf * xrf + Axrs * xrs;
     const T rkr = T(1.0f) / (T(1.0f) + dexp<T>((v + T(55.0f)) / T(75.0f))) * T(1.0f) / (T(1.0f) + dexp<T>((v - T(10.0f)) / T(30.0f)));
     const T GKr = T(0.046);
@@ -453,9 +488,11 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt) {
     const T xs1ss = T(1.0f) / (T(1.0f) + dexp<T>((-(v + T(11.60f))) / T(8.932f)));
     const T txs1 = T(817.3f) + T(1.0f) / (T(2.326e-4f) * dexp<T>((v + T(48.28f)) / T(17.80f)) + T(0.001292f) * dexp<T>((-(v + T(210.0f))) / T(230.0f)));
     xs1 = rl_from<T>(xs1ss, xs1, dt, txs1);
+    st(35, xs1);
     const T xs2ss = xs1ss;
     const T txs2 = T(1.0f) / (T(0.01f) * dexp<T>((v - T(50.0f)) / T(20.0f)) + T(0.0193f) * dexp<T>((-(v + T(66.54f))) / T(31.0f)));
     xs2 = rl_from<T>(xs2ss, xs2, dt, txs2);
+    st(36, xs2);
     const T KsCa = T(1.0f) + T(0.6f) / (T(1.0f) + dpow<T>(T(3.8e-5f) / cai, T(1.4f)));
     const T GKs = T(0.0034);
     const T IKs = GKs * KsCa * xs1 * xs2 * (v - EKs);
@@ -464,6 +501,7 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt) {
     const T xk1ss = T(1.0f) / (T(1.0f) + dexp<T>(-(v + T(2.5538f) * ko + T(144.59f)) / (T(1.5692f) * ko + T(3.8115f))));
     const T txk1 = T(122.2f) / (dexp<T>((-(v + T(127.2f))) / T(20.36f)) + dexp<T>((v + T(236.8f)) / T(69.33f)));
     xk1 = rl_from<T>(xk1ss, xk1, dt, txk1);
+    st(37, xk1);
     const T rk1 = T(1.0f) / (T(1.0f) + dexp<T>((v + T(105.8f) - T(2.6f) * ko) / T(9.493f)));
     const T GK1 = T(0.1908);
     const T IK1 = GK1 * dsqrt<T>(ko) * rk1 * xk1 * (v - EK);
@@ -590,12 +628,14 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt) {
     T tau_rel = bt / (T(1.0f) + T(0.0123f) / cajsr);
     if(tau_rel < T(0.005)) tau_rel = T(0.005);
     Jrelnp = rl_from<T>(Jrel_inf, Jrelnp, dt, tau_rel);
+    st(38, Jrelnp);
     const T btp = T(1.25f) * bt;
     const T a_relp = T(0.5f) * btp;
     const T Jrel_infp = a_relp * (-ICaL) / (T(1.0f) + q8);
     T tau_relp = btp / (T(1.0f) + T(0.0123f) / cajsr);
     if(tau_relp < T(0.005f)) tau_relp = T(0.005f);
     Jrelp = rl_from<T>(Jrel_infp, Jrelp, dt, tau_relp);
+    st(39, Jrelp);
     const T fJ = (T(1.0f) / (T(1.0f) + KmCaMK / CaMKa));
     const T Jrel = (T(1.0f) - fJ) * Jrelnp + fJ * Jrelp;
 
@@ -619,6 +659,8 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt) {
     const T c5 = kmcsqn + cajsr;
     const T Bcajsr = T(1.0f) / (T(1.0f) + csqnmax * kmcsqn / (c5 * c5));
     cajsr += dt * (Bcajsr * (Jtr - Jrel));
+    st(1, nai); st(2, nass); st(3, ki); st(4, kss); st(5, cai); st(6, cass); st(7, cansr); st(8, cajsr);
+    st(40, CaMKt);
 
     y[1] = nai;    y[2] = nass;   y[3] = ki;      y[4] = kss;    y[5] = cai;
     y[6] = cass;   y[7] = cansr;  y[8] = cajsr;   y[9] = m;      y[10] = hf;
