@@ -18,7 +18,8 @@
 namespace mono {
 
 // FP64 path: the device's correctly rounded-or-nearly libm (OCML), so results
-// agree with the reference's glibc to rounding.  FP32 path: the hardware
+// agree with the reference's glibc to rounding (divisions, compiled as a
+// Newton-refined reciprocal, are within 1 ulp).  FP32 path: the hardware
 // approximations (v_exp_f32 / v_log_f32 based, a few ulp), which halve the
 // kernel's instruction count; its accuracy is bounded by the activation-time
 // tests, not by bitwise agreement.
