@@ -150,6 +150,11 @@ int mono_tissue_set_state(mono_tissue *t, const double *host_sv);
 int mono_tissue_get_stats(const mono_tissue *t, mono_stats *out);
 /* 1 = time the ionic and solve phases with HIP events (adds syncs) */
 int mono_tissue_set_timing(mono_tissue *t, int on);
+/* No reference counterpart (diagnostic): how often the solver replayed a
+ * recorded hipGraph of the launches it enqueues ahead of its first status poll,
+ * and how many such graphs it recorded.  Recording is opt-in: MONO_GRAPH=1 in
+ * the environment when the tissue is created. */
+int mono_tissue_graph_counts(const mono_tissue *t, int64_t *launches, int64_t *captures);
 /* device pointers for zero-copy consumers: v (FP64, n), sv (neq rows of
  * `pitch` elements, FP64 or FP32 by precision), b (FP64, n) */
 int mono_tissue_device_ptrs(mono_tissue *t, void **v, void **sv, size_t *pitch, void **b);

@@ -160,6 +160,12 @@ class Tissue:
     def set_timing(self, on: bool) -> None:
         check(lib().mono_tissue_set_timing(self._h, int(on)), "mono_tissue_set_timing")
 
+    def graph_counts(self) -> tuple:
+        """(graph launches, graph recordings) of the solver's pre-poll launch sequences."""
+        launches, captures = C.c_int64(), C.c_int64()
+        check(lib().mono_tissue_graph_counts(self._h, C.byref(launches), C.byref(captures)), "mono_tissue_graph_counts")
+        return launches.value, captures.value
+
     @property
     def v(self) -> np.ndarray:
         out = np.empty(self.n, np.float64)

@@ -21,7 +21,7 @@ TENSOR_UNIFORM, TENSOR_PER_LAYER, TENSOR_PER_CELL = 0, 1, 2
 EXPORTED = (
     "mono_last_error", "mono_device_count", "mono_tissue_create", "mono_tissue_destroy", "mono_tissue_step",
     "mono_tissue_sync", "mono_tissue_num_cells", "mono_tissue_neq", "mono_tissue_get_v", "mono_tissue_set_v",
-    "mono_tissue_get_state", "mono_tissue_set_state", "mono_tissue_save_state", "mono_tissue_restore_state", "mono_tissue_get_stats", "mono_tissue_set_timing",
+    "mono_tissue_get_state", "mono_tissue_set_state", "mono_tissue_save_state", "mono_tissue_restore_state", "mono_tissue_get_stats", "mono_tissue_set_timing", "mono_tissue_graph_counts",
     "mono_tissue_device_ptrs", "mono_ionic_run", "mono_model_neq", "mono_model_initial_state",
     "mono_model_initial_v", "mono_assemble", "mono_cg_stencil", "mono_cg_csr",
     "mono_tissue_observe_setup", "mono_tissue_observe", "mono_tissue_get_activations", "mono_tissue_observe_events", "mono_ionic_run_adaptive",
@@ -94,6 +94,7 @@ def lib() -> C.CDLL:
     L.mono_tissue_restore_state.argtypes = [vp, C.c_char_p]
     L.mono_tissue_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.mono_tissue_set_timing.argtypes = [vp, C.c_int]
+    L.mono_tissue_graph_counts.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.mono_tissue_device_ptrs.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_size_t), C.POINTER(vp)]
     L.mono_ionic_run.argtypes = [C.c_int, C.c_int, vp, i64, vp, C.c_double, C.c_int]
     L.mono_sensor_gradient.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, vp]

@@ -969,6 +969,8 @@ int CgWorkspace::init(int64_t n_, int64_t) {
     MONO_CHECK(hipHostMalloc(&host_sc, 2 * sizeof(CgScalars), hipHostMallocDefault));
     MONO_CHECK(hipMalloc(&sweep_its, sizeof(int) * kMaxBlindSweeps));
     MONO_CHECK(hipHostMalloc(&host_its, sizeof(int) * kMaxBlindSweeps, hipHostMallocDefault));
+    const char *ug = getenv("MONO_GRAPH");
+    use_graphs = ug && ug[0] == '1';
     return 0;
 }
 
@@ -984,12 +986,78 @@ int CgWorkspace::release() {
     if(host_sc) MONO_CHECK(hipHostFree(host_sc));
     if(sweep_its) MONO_CHECK(hipFree(sweep_its));
     if(host_its) MONO_CHECK(hipHostFree(host_its));
+    for(PrefixGraph &g : graphs)
+        if(g.exec) MONO_CHECK(hipGraphExecDestroy(g.exec));
+    graphs.clear();
     sweep_its = nullptr;
     host_its = nullptr;
     last_sched.clear();
     r = p = p2 = q = z = P0 = P1 = nullptr;
     r32 = p32 = p32b = q32 = d32 = nullptr;
     sc = sc_in = host_sc = nullptr;
+    return 0;
+}
+
+template <typename T> static bool same_view(const OpView<T> &a, const OpView<T> &b) {
+    return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.n == b.n && a.pat == b.pat && a.table == b.table &&
+           a.table_int == b.table_int && a.percell == b.percell && a.codes == b.codes && a.ctab == b.ctab && a.active == b.active;
+}
+
+static bool same_key(const CgWorkspace::PrefixKey &a, const CgWorkspace::PrefixKey &b) {
+    const CgConfig &u = a.cfg, &v = b.cfg;
+    return a.kind == b.kind && a.sched == b.sched && same_view(a.op64, b.op64) && same_view(a.op32, b.op32) && a.b == b.b &&
+           a.x == b.x && u.tol == v.tol && u.rel_tol == v.rel_tol && u.max_its == v.max_its && u.jacobi == v.jacobi &&
+           u.mixed == v.mixed && u.inner_rel == v.inner_rel && u.max_outer == v.max_outer && u.poll_chunk == v.poll_chunk;
+}
+
+// First sight of a key: run eagerly and remember it.  Second sight: record the
+// launches into a graph (falling back to eager launches if the recording
+// fails) and launch it.  Later: launch the graph.  `state` is host state the
+// enqueue leaves behind (the mixed solve's buffer parity); a replay restores
+// the value the recording left.
+template <class F> int CgWorkspace::run_prefix(const PrefixKey &key, hipStream_t s, int &state, F &&enqueue) {
+    if(!use_graphs || s == nullptr) return enqueue();
+    PrefixGraph *hit = nullptr;
+    for(PrefixGraph &g : graphs)
+        if(same_key(g.key, key)) hit = &g;
+    if(!hit) {
+        if(graphs.size() >= (size_t)kMaxPrefixGraphs) {
+            size_t old = 0;
+            for(size_t i = 1; i < graphs.size(); i++)
+                if(graphs[i].last_use < graphs[old].last_use) old = i;
+            if(graphs[old].exec) MONO_CHECK(hipGraphExecDestroy(graphs[old].exec));
+            graphs.erase(graphs.begin() + old);
+        }
+        PrefixGraph g;
+        g.key = key;
+        g.last_use = ++graph_clock;
+        graphs.push_back(g);
+        return enqueue();
+    }
+    hit->last_use = ++graph_clock;
+    if(hit->failed) return enqueue();
+    if(!hit->exec) {
+        const int state0 = state;
+        hipGraph_t graph = nullptr;
+        MONO_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue();
+        const hipError_t ec = hipStreamEndCapture(s, &graph);
+        hipError_t ei = hipErrorUnknown;
+        if(rc == 0 && ec == hipSuccess && graph) ei = hipGraphInstantiate(&hit->exec, graph, nullptr, nullptr, 0);
+        if(graph) MONO_CHECK(hipGraphDestroy(graph));
+        if(ei != hipSuccess) {
+            (void)hipGetLastError();
+            hit->exec = nullptr;
+            hit->failed = true;
+            state = state0;
+            return enqueue();
+        }
+        hit->post = state;
+        graph_captures++;
+    }
+    MONO_CHECK(hipGraphLaunch(hit->exec, s));
+    state = hit->post;
+    graph_launches++;
     return 0;
 }
 
@@ -1049,27 +1117,25 @@ int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x
     if(check_pattern(op.pat)) return -1;
     const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, cfg.jacobi, cfg.max_outer, 0};
     double *pb[2] = {p, p2};
-    // residual: r = b - A x, p = r (or z), partials r.r, r.z, b.b -> P0
-    int nres = 0;
-    with_op(op, [&](auto C_, auto P_) {
-        constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
-        if(cfg.jacobi)
-            nres = launch_stencil<double, double, C, P, false>(op, x, nullptr, nullptr, EpiRes64<true>{b, r, pb[0], z}, cs, -1, 0,
-                                                               nullptr, 0, P0, sc, nullptr, s);
-        else
-            nres = launch_stencil<double, double, C, P, false>(op, x, nullptr, nullptr, EpiRes64<false>{b, r, pb[0], z}, cs, -1, 0,
-                                                               nullptr, 0, P0, sc, nullptr, s);
-    });
-    if(nres < 0) return -1;
-    MONO_CHECK(hipGetLastError());
     const double *rz = cfg.jacobi ? z : r;
-    int enq = 0;
-    // the last solve's iteration count includes the kernel that detected
-    // convergence: enqueue that many, so one poll usually sees the solve done
-    int chunk = cfg.poll_chunk > 0 ? cfg.poll_chunk : (last_iters > 2 ? last_iters : 2);
-    for(;;) {
-        for(int j = 0; j < chunk; j++) {
-            const int k = enq + j;
+    int nres = 0;
+    auto residual = [&]() -> int {
+        // residual: r = b - A x, p = r (or z), partials r.r, r.z, b.b -> P0
+        with_op(op, [&](auto C_, auto P_) {
+            constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
+            if(cfg.jacobi)
+                nres = launch_stencil<double, double, C, P, false>(op, x, nullptr, nullptr, EpiRes64<true>{b, r, pb[0], z}, cs, -1, 0,
+                                                                   nullptr, 0, P0, sc, nullptr, s);
+            else
+                nres = launch_stencil<double, double, C, P, false>(op, x, nullptr, nullptr, EpiRes64<false>{b, r, pb[0], z}, cs, -1,
+                                                                   0, nullptr, 0, P0, sc, nullptr, s);
+        });
+        if(nres < 0) return -1;
+        MONO_CHECK(hipGetLastError());
+        return 0;
+    };
+    auto iterations = [&](int k0, int count) -> int {
+        for(int k = k0; k < k0 + count; k++) {
             double *pin = pb[k & 1], *pout = pb[(k + 1) & 1];
             // iteration k: stopping test and beta from P0; p = rz + beta p; q = A p; p.q -> P1
             int nst = 0;
@@ -1083,10 +1149,27 @@ int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x
             k_update<double, double, double><<<grid, kBlock, 0, s>>>(op, x, pout, q, r, z, cfg.jacobi, k, P1, nst, P0, sc, 0);
         }
         MONO_CHECK(hipGetLastError());
-        enq += chunk;
+        return 0;
+    };
+    // the last solve's iteration count includes the kernel that detected
+    // convergence: enqueue that many, so one poll usually sees the solve done
+    int chunk = cfg.poll_chunk > 0 ? cfg.poll_chunk : (last_iters > 2 ? last_iters : 2);
+    PrefixKey key;
+    key.kind = 0;
+    key.sched.assign(1, chunk);
+    key.op64 = op;
+    key.b = b;
+    key.x = x;
+    key.cfg = cfg;
+    int no_state = 0;
+    if(run_prefix(key, s, no_state, [&]() -> int { return residual() ? -1 : iterations(0, chunk); })) return -1;
+    int enq = chunk;
+    for(;;) {
         if(poll(sc, host_sc, s)) return -1;
         if(host_sc->done || enq > cfg.max_its + 2) break;
         chunk = cfg.poll_chunk > 0 ? cfg.poll_chunk : 2;
+        if(iterations(enq, chunk)) return -1;
+        enq += chunk;
     }
     last_iters = host_sc->iters;
     if(st) {
@@ -1147,14 +1230,29 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
     // correction; either way the polled sweeps below finish the solve to the
     // same outer test.  k_note_sweep records what each blind sweep did.
     const int blind = cfg.poll_chunk > 0 ? 0 : (int)last_sched.size();
-    for(int sweep = 0; sweep < blind; sweep++) {
-        const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, 0, cfg.max_outer, sweep};
-        const int nres = residual(sweep, cs);
-        if(nres < 0) return -1;
-        for(int k = 0; k < last_sched[sweep] + 2; k++)
-            if(inner(cs, k, nres)) return -1;
-        k_note_sweep<<<1, 64, 0, s>>>(sc_in, sc, sweep_its + sweep);
-        MONO_CHECK(hipGetLastError());
+    auto blind_sweeps = [&]() -> int {
+        cur = 0;
+        for(int sweep = 0; sweep < blind; sweep++) {
+            const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, 0, cfg.max_outer, sweep};
+            const int nres = residual(sweep, cs);
+            if(nres < 0) return -1;
+            for(int k = 0; k < last_sched[sweep] + 2; k++)
+                if(inner(cs, k, nres)) return -1;
+            k_note_sweep<<<1, 64, 0, s>>>(sc_in, sc, sweep_its + sweep);
+            MONO_CHECK(hipGetLastError());
+        }
+        return 0;
+    };
+    if(blind > 0) {
+        PrefixKey key;
+        key.kind = 1;
+        key.sched = last_sched;
+        key.op64 = op64;
+        key.op32 = op32;
+        key.b = b;
+        key.x = x;
+        key.cfg = cfg;
+        if(run_prefix(key, s, cur, blind_sweeps)) return -1;
     }
     std::vector<int> sched;
     bool blind_read = blind == 0;

@@ -357,6 +357,13 @@ int mono_tissue_set_timing(mono_tissue *t, int on) {
     return 0;
 }
 
+int mono_tissue_graph_counts(const mono_tissue *t, int64_t *launches, int64_t *captures) {
+    if(!t) return -1;
+    if(launches) *launches = t->cg.graph_launches;
+    if(captures) *captures = t->cg.graph_captures;
+    return 0;
+}
+
 static int launch_ionic(mono_tissue *t, double stim_amp) {
     const mono_tissue_desc &d = t->d;
     if(d.precision == MONO_FP64) {
