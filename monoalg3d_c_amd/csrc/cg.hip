@@ -110,14 +110,19 @@ struct CgSetup {
 //   mode 1: after the update of iteration k-1 (P0 = r.r, r.z): beta, stopping test
 //   mode 2: after the mixed-precision outer residual (P0 = r.r, b.b): outer test
 //           and the inner solve's target
-// Returns false (in every thread) when the solve is finished.
+// Returns false (in every thread) when the solve is finished; `now` is then
+// true when this kernel made the decision (false: an earlier kernel did).
 __device__ __forceinline__ bool iter_prologue(int mode, int k, const double *__restrict__ P0, int nprev, CgScalars *sc, CgScalars *so,
-                                              const CgSetup &cs, double &beta) {
+                                              const CgSetup &cs, double &beta, bool &now) {
+    now = false;
     if(mode == 1) {
-        // Block 0 of this very kernel may set `done` while other blocks start;
-        // one read per block keeps the decision uniform across its waves.
+        // Block 0 of this very kernel may set `done` while other blocks start:
+        // only a decision of an earlier kernel (done_iter < k) is taken from
+        // the record, so every block of the deciding kernel decides itself
+        // (the same way, from the same partials).  One read per block keeps
+        // the outcome uniform across its waves.
         __shared__ int s_done;
-        if(threadIdx.x == 0) s_done = sc->done;
+        if(threadIdx.x == 0) s_done = sc->done && sc->done_iter < k;
         __syncthreads();
         if(s_done) return false;
     }
@@ -145,6 +150,8 @@ __device__ __forceinline__ bool iter_prologue(int mode, int k, const double *__r
             sc->iters = 1;
             sc->max_its = cs.max_its;
             sc->beta = 0.0;
+            sc->done_iter = done ? k : 0x7fffffff;
+            __threadfence();
             sc->done = done;
         }
     } else if(mode == 1) {
@@ -157,6 +164,13 @@ __device__ __forceinline__ bool iter_prologue(int mode, int k, const double *__r
             sc->rr_true = rr;
             sc->iters = iters;
             sc->beta = beta;
+            // done_iter is written only with a decision, and before `done`:
+            // a block that sees done = 1 of this kernel never pairs it with a
+            // stale done_iter < k
+            if(done) {
+                sc->done_iter = k;
+                __threadfence();
+            }
             sc->done = done;
         }
     } else {
@@ -180,9 +194,12 @@ __device__ __forceinline__ bool iter_prologue(int mode, int k, const double *__r
             sc->iters = 1;
             sc->max_its = cs.max_its;
             sc->beta = 0.0;
+            sc->done_iter = done ? k : 0x7fffffff;
+            __threadfence();
             sc->done = done;
         }
     }
+    now = done;
     return !done;
 }
 
@@ -496,6 +513,30 @@ __device__ __forceinline__ void tile_sweep(const OpView<TC> &op, const TileGeom 
     }
 }
 
+// x += alpha p over the cells of this block's (tile, plane) items: the
+// deferred correction when a solve ends before the sweep that would apply it.
+template <typename TV, typename TC>
+__device__ __forceinline__ void defer_apply(const OpView<TC> &op, const TileGeom &g, TV *__restrict__ x, const TV *__restrict__ p,
+                                            TV alpha) {
+    const int L = xcd_chunk(blockIdx.x, gridDim.x);
+    const int64_t w0 = (int64_t)L * g.per;
+    const int64_t w1 = w0 + g.per < g.nitems ? w0 + g.per : g.nitems;
+    const int64_t sz = (int64_t)op.nx * op.ny;
+    const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+    for(int64_t w = w0; w < w1; w++) {
+        const int t = (int)(w / op.nz), z = (int)(w % op.nz);
+        const int gx = (t % g.ntx) * kTX + tx, y0 = (t / g.ntx) * kTY;
+        if(gx >= op.nx) continue;
+#pragma unroll
+        for(int q = 0; q < kRPT; q++) {
+            const int gy = y0 + ty + q * (kBlock / kTX);
+            if(gy >= op.ny) continue;
+            const int64_t c = gx + (int64_t)op.nx * gy + sz * z;
+            x[c] = x[c] + alpha * p[c];
+        }
+    }
+}
+
 // Calibration (mode -2): every block records when it started and stays for
 // `spin` ticks of the 100 MHz wall clock, so the host can count how many
 // blocks of this kernel are resident at once.
@@ -611,7 +652,8 @@ __global__ __launch_bounds__(kBlock) void k_stencil2d(OpView<TC> op, int nitems,
                                                       TV *__restrict__ p_out, Epi epi, CgSetup cs, int mode, int k,
                                                       const double *Pin, int nprev, double *Pout, CgScalars *sc, CgScalars *so) {
     double betad = 0.0;
-    if(mode >= 0 && !iter_prologue(mode, k, Pin, nprev, sc, so, cs, betad)) return;
+    bool now;
+    if(mode >= 0 && !iter_prologue(mode, k, Pin, nprev, sc, so, cs, betad, now)) return;
     const TV beta = (TV)betad;
     __shared__ TC ctab[CM == 2 ? kCodes * 7 : 1];
     if constexpr(CM == 2) {
@@ -641,12 +683,25 @@ __global__ __launch_bounds__(kBlock) void k_stencil(OpView<TC> op, TileGeom g, c
         return;
     }
     double betad = 0.0;
-    if(mode >= 0 && !iter_prologue(mode, k, Pin, nprev, sc, so, cs, betad)) return;
+    bool now = false;
+    Epi e = epi;
+    if constexpr(Epi::kDefer) {
+        // x += alpha_{k-1} p_{k-1} of the previous iteration (p_{k-1} is this
+        // kernel's input direction), folded into this sweep
+        e.on = mode == 1;
+        e.alpha = mode == 1 ? (TV)sc->alpha : TV(0);
+    }
+    if(mode >= 0 && !iter_prologue(mode, k, Pin, nprev, sc, so, cs, betad, now)) {
+        // the kernel that ends the solve still owes x its last correction
+        if constexpr(Epi::kDefer)
+            if(now && e.on) defer_apply<TV, TC>(op, g, e.x, x, e.alpha);
+        return;
+    }
     const TV beta = (TV)betad;
     double v[Epi::NR];
 #pragma unroll
     for(int j = 0; j < Epi::NR; j++) v[j] = 0.0;
-    tile_sweep<TV, TC, CM, PAT, FUSE, Epi>(op, g, x, rz, beta, p_out, epi, v);
+    tile_sweep<TV, TC, CM, PAT, FUSE, Epi>(op, g, x, rz, beta, p_out, e, v);
     write_partials<Epi::NR>(v, Pout);
 }
 
@@ -657,7 +712,7 @@ struct NoPre {};
 
 template <typename TV> struct EpiSpmv {
     static constexpr int NR = 1;
-    static constexpr bool kDiag = false, kCorr = false;
+    static constexpr bool kDiag = false, kCorr = false, kDefer = false;
     static constexpr const float *d = nullptr;
     static constexpr const CgScalars *so = nullptr;
     using Pre = NoPre;
@@ -669,10 +724,33 @@ template <typename TV> struct EpiSpmv {
     }
 };
 
+// q = A p, partial p.q, and the previous iteration's x += alpha p_in (its
+// inputs loaded a plane ahead), so the update kernel need not touch x or p
+template <typename TV> struct EpiSpmvX {
+    static constexpr int NR = 1;
+    static constexpr bool kDiag = false, kCorr = false, kDefer = true;
+    static constexpr const float *d = nullptr;
+    static constexpr const CgScalars *so = nullptr;
+    struct Pre {
+        TV x, p;
+    };
+    TV *q;
+    TV *x;
+    const TV *p_in;
+    bool on = false;   // set in the kernel: iterations k >= 1
+    TV alpha = TV(0);  // set in the kernel from the previous update
+    __device__ Pre load(int64_t c) const { return on ? Pre{x[c], p_in[c]} : Pre{TV(0), TV(0)}; }
+    __device__ void operator()(int64_t c, TV ax, TV pc, double, Pre pre, double (&v)[NR]) const {
+        q[c] = ax;
+        if(on) x[c] = pre.x + alpha * pre.p;
+        v[0] += (double)pc * (double)ax;
+    }
+};
+
 // r = b - A x, p = r (or z = D^-1 r, p = z); partial r.r, r.z, b.b
 template <bool JACOBI> struct EpiRes64 {
     static constexpr int NR = 3;
-    static constexpr bool kDiag = JACOBI, kCorr = false;
+    static constexpr bool kDiag = JACOBI, kCorr = false, kDefer = false;
     static constexpr const float *d = nullptr;
     static constexpr const CgScalars *so = nullptr;
     using Pre = double;
@@ -703,7 +781,7 @@ template <bool JACOBI> struct EpiRes64 {
 // buffer: other blocks still read x as their halo).
 template <bool CORR> struct EpiResOuterT {
     static constexpr int NR = 2;  // r.r, b.b
-    static constexpr bool kDiag = false, kCorr = CORR;
+    static constexpr bool kDiag = false, kCorr = CORR, kDefer = false;
     using Pre = double;
     const double *b;
     float *r32;
@@ -728,7 +806,7 @@ constexpr int kIlp = 4;
 template <typename TV, typename TX, typename TC>
 __global__ __launch_bounds__(kBlock) void k_update(OpView<TC> op, TX *__restrict__ x, const TV *__restrict__ p, const TV *__restrict__ q,
                                                    TV *__restrict__ r, TV *__restrict__ z, int jacobi, int k, const double *Pin,
-                                                   int nprev, double *Pout, CgScalars *sc, int fresh) {
+                                                   int nprev, double *Pout, CgScalars *sc, int fresh, int defer) {
     if(sc->done) return;
     double pq[1];
     sum_partials<1>(Pin, nprev, pq);
@@ -741,6 +819,40 @@ __global__ __launch_bounds__(kBlock) void k_update(OpView<TC> op, TX *__restrict
     // fresh: x starts from zero (the mixed solve's correction), so the first
     // iteration writes a p without reading x
     const bool x0 = fresh && k == 0;
+    // defer: x += alpha p is left to the next iteration's stencil sweep
+    if(defer) {
+        int64_t beg, end;
+        chunk_range(op.n, beg, end);
+        double v[2] = {0.0, 0.0};
+        for(int64_t c0 = beg + threadIdx.x; c0 < end; c0 += kIlp * kBlock) {
+            int64_t cc[kIlp];
+            TV qv[kIlp], rv[kIlp];
+#pragma unroll
+            for(int u = 0; u < kIlp; u++) {
+                const int64_t c = c0 + u * kBlock;
+                cc[u] = c < end ? c : end - 1;
+                qv[u] = q[cc[u]];
+                rv[u] = r[cc[u]];
+            }
+#pragma unroll
+            for(int u = 0; u < kIlp; u++) {
+                const int64_t c = c0 + u * kBlock;
+                if(c >= end) continue;
+                const TV rc = rv[u] - a * qv[u];
+                r[c] = rc;
+                if(jacobi) {
+                    double d = diag_of(op, c);
+                    if(d == 0.0) d = 1.0;
+                    const TV zc = (TV)(1.0 / d) * rc;
+                    z[c] = zc;
+                    v[1] += (double)zc * (double)rc;
+                }
+                v[0] += (double)rc * (double)rc;
+            }
+        }
+        write_partials<2>(v, Pout);
+        return;
+    }
     int64_t beg, end;
     chunk_range(op.n, beg, end);
     double v[2] = {0.0, 0.0};
@@ -828,7 +940,8 @@ __global__ __launch_bounds__(kBlock) void k_csr_res(int n, const double *__restr
 __global__ __launch_bounds__(kBlock) void k_csr_dir(int n, double *__restrict__ p, const double *__restrict__ rz, CgSetup cs, int mode,
                                                     int k, const double *Pin, int nprev, CgScalars *sc) {
     double beta = 0.0;
-    if(!iter_prologue(mode, k, Pin, nprev, sc, nullptr, cs, beta)) return;
+    bool now;
+    if(!iter_prologue(mode, k, Pin, nprev, sc, nullptr, cs, beta, now)) return;
     int64_t beg, end;
     chunk_range(n, beg, end);
     for(int64_t c = beg + threadIdx.x; c < end; c += kBlock) p[c] = rz[c] + beta * p[c];
@@ -1118,6 +1231,11 @@ int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x
     const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, cfg.jacobi, cfg.max_outer, 0};
     double *pb[2] = {p, p2};
     const double *rz = cfg.jacobi ? z : r;
+    // boxes of more than one plane (the tiled stencil kernel): each
+    // iteration's x += alpha p is applied by the next iteration's stencil
+    // sweep, which reads p anyway (72 instead of 80 bytes per cell and
+    // iteration); the sweep that ends the solve applies the last one
+    const bool defer = op.nz > 1;
     int nres = 0;
     auto residual = [&]() -> int {
         // residual: r = b - A x, p = r (or z), partials r.r, r.z, b.b -> P0
@@ -1141,12 +1259,17 @@ int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x
             int nst = 0;
             with_op(op, [&](auto C_, auto P_) {
                 constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
-                nst = launch_stencil<double, double, C, P, true>(op, pin, rz, pout, EpiSpmv<double>{q}, cs, k == 0 ? 0 : 1, k, P0,
-                                                                 k == 0 ? nres : grid, P1, sc, nullptr, s);
+                if(defer)
+                    nst = launch_stencil<double, double, C, P, true>(op, pin, rz, pout, EpiSpmvX<double>{q, x, pin}, cs,
+                                                                     k == 0 ? 0 : 1, k, P0, k == 0 ? nres : grid, P1, sc, nullptr, s);
+                else
+                    nst = launch_stencil<double, double, C, P, true>(op, pin, rz, pout, EpiSpmv<double>{q}, cs, k == 0 ? 0 : 1, k,
+                                                                     P0, k == 0 ? nres : grid, P1, sc, nullptr, s);
             });
             if(nst < 0) return -1;
-            // alpha from P1; x += alpha p; r -= alpha q; r.r, r.z -> P0
-            k_update<double, double, double><<<grid, kBlock, 0, s>>>(op, x, pout, q, r, z, cfg.jacobi, k, P1, nst, P0, sc, 0);
+            // alpha from P1; x += alpha p (unless deferred); r -= alpha q; r.r, r.z -> P0
+            k_update<double, double, double><<<grid, kBlock, 0, s>>>(op, x, pout, q, r, z, cfg.jacobi, k, P1, nst, P0, sc, 0,
+                                                                     defer ? 1 : 0);
         }
         MONO_CHECK(hipGetLastError());
         return 0;
@@ -1220,7 +1343,7 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
                                                            k == 0 ? nres : grid, P1, sc_in, sc, s);
         });
         if(nst < 0) return -1;
-        k_update<float, float, float><<<grid, kBlock, 0, s>>>(op32, d32, pout, q32, r32, nullptr, 0, k, P1, nst, P0, sc_in, 1);
+        k_update<float, float, float><<<grid, kBlock, 0, s>>>(op32, d32, pout, q32, r32, nullptr, 0, k, P1, nst, P0, sc_in, 1, 0);
         return 0;
     };
     // Sweeps enqueued without polling, as many as the previous solve needed,

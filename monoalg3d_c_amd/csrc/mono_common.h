@@ -49,7 +49,7 @@ struct CgScalars {
     int32_t iters;    // iteration count as the reference counts it (starts at 1)
     int32_t max_its;
     int32_t done;     // 1 once converged or max_its reached
-    int32_t pad;
+    int32_t done_iter;  // iteration (kernel index k) at which `done` was decided
 };
 
 }  // namespace mono
