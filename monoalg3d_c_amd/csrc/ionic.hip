@@ -63,12 +63,20 @@ template <typename T> __device__ __forceinline__ double cell_stim(const IonicLau
     return st;
 }
 
+// The state rows, v and b stream through the step kernel once per step with
+// no reuse inside it: they are read and written with the non-temporal hint,
+// so they do not displace the solver's vectors from the caches (measured at
+// 16M cells: FP32 ORd kernel 1.10 -> 0.93 ms, and the solve that follows
+// slightly faster).
+template <typename T> __device__ __forceinline__ T stream_load(const T *p) { return __builtin_nontemporal_load(p); }
+template <typename T> __device__ __forceinline__ void stream_store(T *p, T v) { __builtin_nontemporal_store(v, p); }
+
 // Writes each new ORd state to its HBM row as soon as the model has it final.
 template <typename T> struct RowStore {
     T *sv;
     size_t pitch;
     int64_t i;
-    __device__ __forceinline__ void operator()(int s, T val) const { sv[(size_t)s * pitch + i] = val; }
+    __device__ __forceinline__ void operator()(int s, T val) const { stream_store(&sv[(size_t)s * pitch + i], val); }
 };
 
 // ONE: the launch advances one ODE step per PDE step (num_steps == 1, every
@@ -87,15 +95,15 @@ __device__ __forceinline__ void ionic_step_body(const IonicLaunch<T> &a) {
 
     T y[NEQ];
 #pragma unroll
-    for(int s = 1; s < NEQ; s++) y[s] = a.sv[(size_t)s * a.pitch + i];
-    double V = a.first_step ? (double)a.sv[i] : a.v[i];
+    for(int s = 1; s < NEQ; s++) y[s] = stream_load(&a.sv[(size_t)s * a.pitch + i]);
+    double V = a.first_step ? (double)a.sv[i] : stream_load(&a.v[i]);
 
     const double st = cell_stim(a, i);
     if constexpr(ONE && MODEL == kORd2011) {
         y[0] = T(V);
         V = V + (double)ord2011_step<T>(y, T(st), a.dt, RowStore<T>{a.sv, a.pitch, i});
-        if(a.write_vm_row) a.sv[i] = T(V);
-        if(a.b) a.b[i] = V * a.alpha;
+        if(a.write_vm_row) stream_store(&a.sv[i], T(V));
+        if(a.b) stream_store(&a.b[i], V * a.alpha);
         return;
     }
     float scale[4] = {1.f, 1.f, 1.f, 1.f};
@@ -109,10 +117,10 @@ __device__ __forceinline__ void ionic_step_body(const IonicLaunch<T> &a) {
         V = V + (double)dv;
     }
 #pragma unroll
-    for(int s = 1; s < NEQ; s++) a.sv[(size_t)s * a.pitch + i] = y[s];
-    if(a.write_vm_row) a.sv[i] = T(V);
+    for(int s = 1; s < NEQ; s++) stream_store(&a.sv[(size_t)s * a.pitch + i], y[s]);
+    if(a.write_vm_row) stream_store(&a.sv[i], T(V));
     // fused UPDATE_MONODOMAIN: b = Vm * alpha
-    if(a.b) a.b[i] = V * a.alpha;
+    if(a.b) stream_store(&a.b[i], V * a.alpha);
 }
 
 // Adaptive forward Euler (solve_forward_euler_cpu_adpt, default_solvers.c:61-176)
