@@ -87,6 +87,40 @@ def test_cg_stencil_matches_oracle(oracle, aniso, jacobi):
     assert (r @ r) <= 1e-18
 
 
+@pytest.mark.parametrize("max_its", [1, 2, 5])
+def test_cg_stencil_iteration_cap_matches_oracle(oracle, max_its):
+    """A solve stopped by max_its (linear_system_solver.c:159-163) returns the
+    iterate after exactly that many steps; on boxes of several planes the last
+    x += alpha p is applied by the kernel that ends the solve."""
+    import monoalg3d_c_amd as m
+    announce(f"cg stencil max_its={max_its}")
+    shape = (10, 9, 6)
+    n = 10 * 9 * 6
+    coef, a = m.assemble(shape, aniso=True, tensor=_fibres(shape), tensor_source=m.TENSOR_PER_LAYER)
+    rng = np.random.default_rng(8)
+    b = rng.normal(size=n) * a * 80.0
+    x0 = np.full(n, -80.0)
+    xr, itr, er = oracle.cg_stencil(shape, coef, b, x0, 1e-30, max_its, False)
+    xg, itg, eg = m.cg_stencil(shape, coef, b, x0, tol=1e-30, max_its=max_its)
+    assert itg == itr == max_its
+    assert np.abs(xg - xr).max() <= 1e-10 * np.abs(xr).max()
+    assert abs(eg - er) <= 1e-8 * er
+
+
+def test_cg_stencil_converged_start(oracle):
+    """x0 already solves the system: no iteration, x unchanged."""
+    import monoalg3d_c_amd as m
+    announce("cg stencil converged start")
+    shape = (10, 9, 6)
+    n = 10 * 9 * 6
+    coef, a = m.assemble(shape, tensor=m.iso_tensor(0.00013))
+    x0 = np.random.default_rng(9).normal(size=n) - 80.0
+    b = oracle.stencil_apply(shape, coef, x0)
+    xg, itg, eg = m.cg_stencil(shape, coef, b, x0, tol=1e-6, max_its=50)
+    assert itg == 1
+    np.testing.assert_array_equal(xg, x0)
+
+
 def test_cg_mixed_reaches_fp64_tolerance(oracle):
     import monoalg3d_c_amd as m
     announce("cg stencil mixed precision")
