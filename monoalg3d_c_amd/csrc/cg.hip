@@ -1157,7 +1157,11 @@ template <class F> int CgWorkspace::run_prefix(const PrefixKey &key, hipStream_t
     if(!hit->exec) {
         const int state0 = state;
         hipGraph_t graph = nullptr;
-        MONO_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        if(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            hit->failed = true;
+            return enqueue();
+        }
         const int rc = enqueue();
         const hipError_t ec = hipStreamEndCapture(s, &graph);
         hipError_t ei = hipErrorUnknown;
