@@ -273,9 +273,13 @@ double model_initial_v(int model) {
     return 0.00000820413566106744;
 }
 
+// FP64 step kernels run one wave per block: the work per cell is long (VALU
+// bound), and single-wave blocks fill the SIMDs more evenly than 4-wave ones
+// (16M-cell ORd: 2.46 ms at 64 threads, 2.47 at 128, 2.54 at 256, 2.82 at 512).
+// The HBM-bound FP32 kernels are fastest at 256 (0.93 ms; 1.02 at 128, 1.24 at 512).
 template <bool ONE>
 static hipError_t launch_step64(int model, const IonicLaunch<double> &a, hipStream_t s) {
-    const int block = 256;
+    const int block = 64;
     const unsigned grid = (unsigned)((a.n + block - 1) / block);
     switch(model) {
     case kTT2006: k_ionic_step64<kTT2006, ONE><<<grid, block, 0, s>>>(a); break;
