@@ -110,7 +110,12 @@ typedef struct mono_tissue_desc {
      * adaptive Euler solver with min dt = dt_ode and max dt = dt_pde; each step
      * advances every cell to t + dt_ode with its own step size */
     int ode_adaptive;
-    double ode_abs_tol, ode_rel_tol;  /* [ode_solver] abs_tol / rel_tol (reference defaults 1e-5) */
+    double ode_abs_tol, ode_rel_tol;  /* [ode_solver] abstol / reltol (reference defaults 1e-5) */
+    /* > 0: the dt of ALPHA when it differs from the loop's dt_pde.  The
+     * reference raises the loop's dt_pde to dt_ode when dt_ode is larger
+     * (monodomain_solver.c:512-519) while ALPHA keeps the configured dt_pde
+     * (the_monodomain_solver->dt, monodomain_solver.c:1326); 0 = dt_pde. */
+    double alpha_dt;
 } mono_tissue_desc;
 
 typedef struct mono_stats {
@@ -193,6 +198,18 @@ int mono_tissue_get_activations(mono_tissue *t, int32_t *n_act, float *act, int3
 /* max_events of the observer (set up, or restored with a checkpoint); 0 when none */
 int mono_tissue_observe_events(const mono_tissue *t);
 
+/* First-action-potential tracker for precision studies (no reference
+ * counterpart; save_mesh.c:754-913 records APD at a fixed threshold instead).
+ * setup records every cell's current Vm as its resting potential; each
+ * mono_tissue_apd_sample (after mono_tissue_step) takes one sample of Vm at
+ * the current time.  Per cell: lat = first upward crossing of
+ * activation_threshold, linearly interpolated between samples; vmax = the
+ * peak after it; apd90 = time from lat to the downward crossing of
+ * V90 = vmax - 0.9 (vmax - vrest), interpolated.  NaN where absent. */
+int mono_tissue_apd_setup(mono_tissue *t, double activation_threshold);
+int mono_tissue_apd_sample(mono_tissue *t);
+int mono_tissue_apd_get(mono_tissue *t, double *lat, double *vmax, double *apd90);
+
 /* Standalone ionic solve: n cells, states AoS (n x neq, FP64, in/out), per-cell
  * stimulus (n doubles, may be NULL), nsteps ODE steps of dt.  precision selects
  * the FP64 or FP32 kernel. */
@@ -222,6 +239,21 @@ int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, const int64_t
 int mono_ionic_run_adaptive(int model, int precision, double *states, int64_t n, const double *stim, double current_t,
                             double *ode_dt, double *ode_time_new, double *ode_previous_dt, double min_dt, double max_dt,
                             double abs_tol, double rel_tol);
+/* GPU side of the ODE-solver plugin ABI (the reference core's use_gpu path):
+ *   mono_ode_gpu_alloc  SET_ODE_INITIAL_CONDITIONS_GPU set_model_initial_conditions_gpu
+ *                       (ode_solver.h:46; e.g. ten_tusscher_2006_RS_GPU.cu:8-40):
+ *                       allocates the FP64 state of n cells on the device as
+ *                       neq rows of *pitch_bytes (row 0 = Vm), every cell at
+ *                       the model's default initial state.
+ *   mono_ode_gpu_solve  SOLVE_MODEL_ODES solve_model_odes_gpu (ode_solver.h:37;
+ *                       e.g. ten_tusscher_2006_RS_GPU.cu:42-156): num_steps steps
+ *                       of dt for every cell, V read from and written back to
+ *                       row 0, stim a host array of n (NULL: none).
+ *   mono_device_free    the core's free of a GPU sv (ode_solver.c:149-158). */
+int mono_ode_gpu_alloc(int model, int64_t n, double **sv, size_t *pitch_bytes);
+int mono_ode_gpu_solve(int model, double *sv, size_t pitch_bytes, int64_t n, const double *stim, double dt, int num_steps);
+int mono_device_free(void *p);
+
 int mono_model_neq(int model);
 /* default initial state (neq doubles) and INITIAL_V of a model */
 int mono_model_initial_state(int model, double *out);
@@ -238,9 +270,38 @@ int mono_assemble(const mono_tissue_desc *desc, double *coef27, double *alpha_ou
 int mono_cg_stencil(int nx, int ny, int nz, const double *coef27, const uint8_t *active, const double *b,
                     double *x, double tol, double rel_tol, int max_its, int jacobi, int precision,
                     int *iterations, double *error);
-/* CG on a CSR matrix (host arrays), FP64. */
+/* CG on a CSR matrix (host arrays), FP64: create + solve + destroy below. */
 int mono_cg_csr(int n, const int64_t *row_ptr, const int32_t *col, const double *val, const double *b,
                 double *x, double tol, int max_its, int jacobi, int *iterations, double *error);
+
+/* A CSR operator resident on the device between solves.
+ *   mono_csr_create   INIT_LINEAR_SYSTEM (linear_system_solver_config.h:18) as
+ *                     init_gpu_conjugate_gradient does it (gpu_solvers_cublas_12.c:46-119):
+ *                     the matrix (rows as grid_to_csr builds them, grid.c:573-649)
+ *                     is uploaded once; the Jacobi diagonal is taken on the device.
+ *   mono_csr_solve    SOLVE_LINEAR_SYSTEM conjugate_gradient (linear_system_solver.c:47-200):
+ *                     x (host, n) is the warm start and the result; b (host, n), or
+ *                     NULL for the right-hand side mono_csr_update_rhs left on the
+ *                     device.  Only b and x cross PCIe.
+ *   mono_csr_set_alpha / mono_csr_update_rhs
+ *                     UPDATE_MONODOMAIN update_monodomain_default
+ *                     (update_monodomain_config.h:15-19, update_monodomain.c:19-56):
+ *                     b = alpha * Vm formed on the device, for the next mono_csr_solve.
+ *   mono_csr_destroy  END_LINEAR_SYSTEM (linear_system_solver_config.h:21). */
+typedef struct mono_csr mono_csr;
+int mono_csr_create(int n, const int64_t *row_ptr, const int32_t *col, const double *val, mono_csr **out);
+int mono_csr_num_rows(const mono_csr *A);
+int mono_csr_set_alpha(mono_csr *A, const double *alpha);
+int mono_csr_update_rhs(mono_csr *A, const double *vm);
+int mono_csr_get_rhs(mono_csr *A, double *b);
+int mono_csr_solve(mono_csr *A, const double *b, double *x, double tol, int max_its, int jacobi, int *iterations,
+                   double *error);
+int mono_csr_destroy(mono_csr *A);
+
+/* UPDATE_MONODOMAIN update_monodomain_default (update_monodomain.c:19-56) on host
+ * arrays: b[i] = vm[i] * alpha[i], alpha = ALPHA(beta, cm, dt_pde, dx, dy, dz)
+ * of each cell (constants.h:10-11). */
+int mono_update_rhs(int64_t n, const double *alpha, const double *vm, double *b);
 
 #ifdef __cplusplus
 }

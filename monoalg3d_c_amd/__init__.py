@@ -75,16 +75,41 @@ def layered_fibre_tensors(nz: int, theta0_deg: float, theta1_deg: float, sigma_l
     return out
 
 
+def splitmix64_uniform(seed: int, index: np.ndarray) -> np.ndarray:
+    """U[0, 1) draws keyed by (seed, index): SplitMix64 (Steele, Lea & Flood,
+    "Fast splittable pseudorandom number generators", OOPSLA 2014) of
+    seed + (index + 1) * 0x9E3779B97F4A7C15, top 53 bits."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (np.asarray(index, np.uint64) + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def fibrosis_mask(shape, fraction: float, seed: int, chunk: int = 1 << 22) -> np.ndarray:
+    """Conducting cells (1) of a box with Bernoulli(fraction) fibrotic cells (0),
+    cell c fibrotic when its SplitMix64 draw keyed by (seed, c) is below
+    fraction (c lexicographic, x fastest)."""
+    n = int(np.prod(shape))
+    out = np.empty(n, np.uint8)
+    for c0 in range(0, n, chunk):
+        idx = np.arange(c0, min(n, c0 + chunk), dtype=np.uint64)
+        out[c0:c0 + len(idx)] = splitmix64_uniform(seed, idx) >= fraction
+    return out
+
+
 def make_desc(shape, h=(100.0, 100.0, 100.0), model=MODEL_TT2006, precision=FP64, dt=0.02, dt_ode=None, beta=0.14,
               cm=1.0, aniso=False, tensor=None, tensor_source=TENSOR_UNIFORM, g_cell=None, active=None,
               cg_tol=1e-16, cg_rel_tol=0.0, cg_max_its=200, jacobi=False, inner_rel=1e-4,
-              stim=(0.0, 2.0, -50.0, 500.0, 0.0), stim_mask=None, ionic_scale=None, adaptive=None):
+              stim=(0.0, 2.0, -50.0, 500.0, 0.0), stim_mask=None, ionic_scale=None, adaptive=None, alpha_dt=0.0):
     """Build a TissueDesc; returns (desc, keepalive) where keepalive holds the arrays.
 
     stim_mask (n,) 0/1 replaces the x < stim_x_limit rule; ionic_scale (n, 4)
     float32 scales TT2006 GKr/GCaL/GKs or MS2003 tau_close per cell.
     adaptive: None, or (abs_tol, rel_tol) for the adaptive Euler ODE solver
-    (min dt = dt_ode, max dt = dt; CRN 1998)."""
+    (min dt = dt_ode, max dt = dt; CRN 1998).  alpha_dt > 0: the dt of ALPHA
+    when it differs from the loop's dt (a dt_ode above dt_pde)."""
     d = TissueDesc()
     d.nx, d.ny, d.nz = shape
     d.dx, d.dy, d.dz = h
@@ -124,6 +149,7 @@ def make_desc(shape, h=(100.0, 100.0, 100.0), model=MODEL_TT2006, precision=FP64
         d.ode_abs_tol, d.ode_rel_tol = adaptive
     d.cg_tol, d.cg_rel_tol, d.cg_max_its, d.cg_jacobi = cg_tol, cg_rel_tol, cg_max_its, int(jacobi)
     d.inner_rel = inner_rel
+    d.alpha_dt = alpha_dt
     d.stim_start, d.stim_duration, d.stim_current, d.stim_x_limit, d.stim_period = stim
     return d, keep
 
@@ -225,6 +251,22 @@ class Tissue:
         """One save call at the current time (every print_rate steps, before stepping)."""
         check(lib().mono_tissue_observe(self._h), "mono_tissue_observe")
 
+    # first-action-potential tracker (activation time, peak, APD90)
+    def apd_setup(self, activation_threshold=-30.0) -> None:
+        """Record the current Vm as each cell's resting potential and start
+        tracking its first action potential (sample after every step)."""
+        check(lib().mono_tissue_apd_setup(self._h, activation_threshold), "mono_tissue_apd_setup")
+
+    def apd_sample(self) -> None:
+        check(lib().mono_tissue_apd_sample(self._h), "mono_tissue_apd_sample")
+
+    def apd(self) -> dict:
+        """{'lat', 'vmax', 'apd90'}: (n,) float64, NaN where absent."""
+        out = {k: np.empty(self.n, np.float64) for k in ("lat", "vmax", "apd90")}
+        check(lib().mono_tissue_apd_get(self._h, ptr(out["lat"]), ptr(out["vmax"]), ptr(out["apd90"])),
+              "mono_tissue_apd_get")
+        return out
+
     def activations(self) -> dict:
         n, k = self.n, lib().mono_tissue_observe_events(self._h)
         n_act = np.zeros(n, np.int32)
@@ -260,8 +302,13 @@ def ionic_run(model: int, states: np.ndarray, stim: np.ndarray | None, dt: float
               precision: int = FP64) -> np.ndarray:
     """SOLVE_MODEL_ODES on AoS states (n, neq); returns the new states."""
     s = np.array(states, np.float64, copy=True, order="C")
+    neq = model_neq(model)
+    if s.ndim != 2 or s.shape[1] != neq:
+        raise ValueError(f"states must have shape (n, {neq}) for model {model}, got {s.shape}")
     n = s.shape[0]
-    st = None if stim is None else np.ascontiguousarray(stim, np.float64)
+    st = None if stim is None else np.ascontiguousarray(stim, np.float64).reshape(-1)
+    if st is not None and st.size != n:
+        raise ValueError(f"stim has {st.size} entries for {n} cells")
     check(lib().mono_ionic_run(model, precision, ptr(s), n, ptr(st), dt, nsteps), "mono_ionic_run")
     return s
 
@@ -271,10 +318,18 @@ def ionic_run_adaptive(model: int, states: np.ndarray, aux: np.ndarray, stim, cu
     """One adaptive-Euler call (to current_t + min_dt) on AoS states (n, neq) and
     per-cell aux (n, 3) = (dt, time_new, previous_dt); returns new (states, aux)."""
     s = np.array(states, np.float64, copy=True, order="C")
+    neq = model_neq(model)
+    if s.ndim != 2 or s.shape[1] != neq:
+        raise ValueError(f"states must have shape (n, {neq}) for model {model}, got {s.shape}")
     n = s.shape[0]
-    a = np.array(aux, np.float64, copy=True).reshape(n, 3)
+    a = np.array(aux, np.float64, copy=True)
+    if a.size != 3 * n:
+        raise ValueError(f"aux must have shape ({n}, 3), got {a.shape}")
+    a = a.reshape(n, 3)
     dt, tn, pdt = (np.ascontiguousarray(a[:, k]) for k in range(3))
-    st = None if stim is None else np.ascontiguousarray(stim, np.float64)
+    st = None if stim is None else np.ascontiguousarray(stim, np.float64).reshape(-1)
+    if st is not None and st.size != n:
+        raise ValueError(f"stim has {st.size} entries for {n} cells")
     check(lib().mono_ionic_run_adaptive(model, precision, ptr(s), n, ptr(st), current_t, ptr(dt), ptr(tn), ptr(pdt), min_dt,
                                         max_dt, abs_tol, rel_tol), "mono_ionic_run_adaptive")
     return s, np.stack([dt, tn, pdt], axis=1)
@@ -294,26 +349,113 @@ def cg_stencil(shape, coef27, b, x0, tol=1e-16, rel_tol=0.0, max_its=200, jacobi
     """CG on an (n, 27) stencil operator; returns (x, iterations, error)."""
     nx, ny, nz = shape
     n = nx * ny * nz
-    c = np.ascontiguousarray(np.asarray(coef27, np.float64).reshape(n, 27).T)
-    bb = np.ascontiguousarray(b, np.float64)
-    x = np.array(x0, np.float64, copy=True)
-    act = None if active is None else np.ascontiguousarray(active, np.uint8)
+    coef27 = np.asarray(coef27, np.float64)
+    if coef27.size != 27 * n:
+        raise ValueError(f"coef27 must have shape ({n}, 27), got {coef27.shape}")
+    c = np.ascontiguousarray(coef27.reshape(n, 27).T)
+    bb = np.ascontiguousarray(b, np.float64).reshape(-1)
+    x = np.array(x0, np.float64, copy=True).reshape(-1)
+    act = None if active is None else np.ascontiguousarray(active, np.uint8).reshape(-1)
+    if bb.size != n or x.size != n or (act is not None and act.size != n):
+        raise ValueError(f"b, x0 and active must have {n} entries")
     its, err = C.c_int(), C.c_double()
     check(lib().mono_cg_stencil(nx, ny, nz, ptr(c), ptr(act), ptr(bb), ptr(x), tol, rel_tol, max_its, int(jacobi),
                                 precision, C.byref(its), C.byref(err)), "mono_cg_stencil")
     return x, its.value, err.value
 
 
+def _check_csr(rp, cc, vv, n_vec):
+    """Lengths of a CSR operator and its vectors (the C side checks the
+    row_ptr order and column range)."""
+    n = len(rp) - 1
+    if n < 1:
+        raise ValueError("row_ptr needs at least two entries")
+    if rp[0] != 0 or np.any(np.diff(rp) < 0):
+        raise ValueError("row_ptr must start at 0 and never decrease")
+    if len(cc) != rp[-1] or len(vv) != rp[-1]:
+        raise ValueError(f"col and val must have row_ptr[-1] = {rp[-1]} entries (got {len(cc)}, {len(vv)})")
+    for k, m in n_vec.items():
+        if m != n:
+            raise ValueError(f"{k} has {m} entries for {n} rows")
+    return n
+
+
 def cg_csr(row_ptr, col, val, b, x0, tol=1e-16, max_its=200, jacobi=False):
-    rp = np.ascontiguousarray(row_ptr, np.int64)
-    cc = np.ascontiguousarray(col, np.int32)
-    vv = np.ascontiguousarray(val, np.float64)
-    bb = np.ascontiguousarray(b, np.float64)
-    x = np.array(x0, np.float64, copy=True)
+    """SOLVE_LINEAR_SYSTEM on a CSR matrix (upload, solve, free); returns (x, iterations, error)."""
+    rp = np.ascontiguousarray(row_ptr, np.int64).reshape(-1)
+    cc = np.ascontiguousarray(col, np.int32).reshape(-1)
+    vv = np.ascontiguousarray(val, np.float64).reshape(-1)
+    bb = np.ascontiguousarray(b, np.float64).reshape(-1)
+    x = np.array(x0, np.float64, copy=True).reshape(-1)
+    _check_csr(rp, cc, vv, {"b": bb.size, "x0": x.size})
     its, err = C.c_int(), C.c_double()
     check(lib().mono_cg_csr(len(rp) - 1, ptr(rp), ptr(cc), ptr(vv), ptr(bb), ptr(x), tol, max_its, int(jacobi),
                             C.byref(its), C.byref(err)), "mono_cg_csr")
     return x, its.value, err.value
+
+
+class Csr:
+    """A CSR operator kept on the device between solves (INIT_LINEAR_SYSTEM /
+    SOLVE_LINEAR_SYSTEM / END_LINEAR_SYSTEM with persistent data)."""
+
+    def __init__(self, row_ptr, col, val):
+        self._rp = np.ascontiguousarray(row_ptr, np.int64).reshape(-1)
+        cc = np.ascontiguousarray(col, np.int32).reshape(-1)
+        vv = np.ascontiguousarray(val, np.float64).reshape(-1)
+        self.n = _check_csr(self._rp, cc, vv, {})
+        self._h = C.c_void_p()
+        check(lib().mono_csr_create(self.n, ptr(self._rp), ptr(cc), ptr(vv), C.byref(self._h)), "mono_csr_create")
+
+    def set_alpha(self, alpha) -> None:
+        a = np.ascontiguousarray(alpha, np.float64).reshape(-1)
+        if a.size != self.n:
+            raise ValueError(f"alpha has {a.size} entries for {self.n} rows")
+        check(lib().mono_csr_set_alpha(self._h, ptr(a)), "mono_csr_set_alpha")
+
+    def update_rhs(self, vm) -> None:
+        """b = alpha * vm on the device (UPDATE_MONODOMAIN); the next solve(b=None) uses it."""
+        v = np.ascontiguousarray(vm, np.float64).reshape(-1)
+        if v.size != self.n:
+            raise ValueError(f"vm has {v.size} entries for {self.n} rows")
+        check(lib().mono_csr_update_rhs(self._h, ptr(v)), "mono_csr_update_rhs")
+
+    def rhs(self) -> np.ndarray:
+        out = np.empty(self.n, np.float64)
+        check(lib().mono_csr_get_rhs(self._h, ptr(out)), "mono_csr_get_rhs")
+        return out
+
+    def solve(self, b, x0, tol=1e-16, max_its=200, jacobi=False):
+        """Returns (x, iterations, error); b=None uses the device right-hand side."""
+        bb = None if b is None else np.ascontiguousarray(b, np.float64).reshape(-1)
+        x = np.array(x0, np.float64, copy=True).reshape(-1)
+        if x.size != self.n or (bb is not None and bb.size != self.n):
+            raise ValueError(f"b and x0 must have {self.n} entries")
+        its, err = C.c_int(), C.c_double()
+        check(lib().mono_csr_solve(self._h, ptr(bb), ptr(x), tol, max_its, int(jacobi), C.byref(its), C.byref(err)),
+              "mono_csr_solve")
+        return x, its.value, err.value
+
+    def close(self) -> None:
+        if self._h:
+            check(lib().mono_csr_destroy(self._h), "mono_csr_destroy")
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def update_rhs(alpha, vm) -> np.ndarray:
+    """UPDATE_MONODOMAIN on host arrays: b = alpha * vm (computed on the device)."""
+    a = np.ascontiguousarray(alpha, np.float64).reshape(-1)
+    v = np.ascontiguousarray(vm, np.float64).reshape(-1)
+    if a.size != v.size:
+        raise ValueError("alpha and vm must have the same length")
+    b = np.empty(v.size, np.float64)
+    check(lib().mono_update_rhs(v.size, ptr(a), ptr(v), ptr(b)), "mono_update_rhs")
+    return b
 
 
 def device_count() -> int:

@@ -25,7 +25,9 @@ EXPORTED = (
     "mono_tissue_device_ptrs", "mono_ionic_run", "mono_model_neq", "mono_model_initial_state",
     "mono_model_initial_v", "mono_assemble", "mono_cg_stencil", "mono_cg_csr",
     "mono_tissue_observe_setup", "mono_tissue_observe", "mono_tissue_get_activations", "mono_tissue_observe_events", "mono_ionic_run_adaptive",
-    "mono_sensor_gradient",
+    "mono_sensor_gradient", "mono_csr_create", "mono_csr_num_rows", "mono_csr_set_alpha", "mono_csr_update_rhs",
+    "mono_csr_get_rhs", "mono_csr_solve", "mono_csr_destroy", "mono_update_rhs", "mono_ode_gpu_alloc",
+    "mono_ode_gpu_solve", "mono_device_free", "mono_tissue_apd_setup", "mono_tissue_apd_sample", "mono_tissue_apd_get",
 )
 
 
@@ -48,6 +50,7 @@ class TissueDesc(C.Structure):
         ("stim_x_limit", C.c_double), ("stim_period", C.c_double),
         ("stim_mask", C.c_void_p), ("ionic_scale", C.c_void_p),
         ("ode_adaptive", C.c_int), ("ode_abs_tol", C.c_double), ("ode_rel_tol", C.c_double),
+        ("alpha_dt", C.c_double),
     ]
 
 
@@ -111,6 +114,20 @@ def lib() -> C.CDLL:
     L.mono_tissue_get_activations.argtypes = [vp, vp, vp, vp, vp]
     L.mono_tissue_observe_events.argtypes = [vp]
     L.mono_cg_csr.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
+    L.mono_csr_create.argtypes = [C.c_int, vp, vp, vp, C.POINTER(vp)]
+    L.mono_csr_num_rows.argtypes = [vp]
+    L.mono_csr_set_alpha.argtypes = [vp, vp]
+    L.mono_csr_update_rhs.argtypes = [vp, vp]
+    L.mono_csr_get_rhs.argtypes = [vp, vp]
+    L.mono_csr_solve.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
+    L.mono_csr_destroy.argtypes = [vp]
+    L.mono_update_rhs.argtypes = [i64, vp, vp, vp]
+    L.mono_ode_gpu_alloc.argtypes = [C.c_int, i64, C.POINTER(vp), C.POINTER(C.c_size_t)]
+    L.mono_ode_gpu_solve.argtypes = [C.c_int, vp, C.c_size_t, i64, vp, C.c_double, C.c_int]
+    L.mono_device_free.argtypes = [vp]
+    L.mono_tissue_apd_setup.argtypes = [vp, C.c_double]
+    L.mono_tissue_apd_sample.argtypes = [vp]
+    L.mono_tissue_apd_get.argtypes = [vp, vp, vp, vp]
     _lib = L
     return L
 

@@ -16,8 +16,14 @@ step of the atlas advances every instance by one step of its own run:
     stim_if_x_less_than, stimuli_library/stimuli.c);
   - one CG solves all blocks at once.  It stops when the atlas residual r.r is
     below the tolerance, which bounds every instance's own r.r by the same
-    tolerance, so each instance is solved at least as tightly as its
-    stand-alone run.
+    tolerance, so with the reference's absolute test (`tolerance`) each
+    instance is solved at least as tightly as its stand-alone run.  A relative
+    test (cg_rel_tol, ||r|| <= rel ||b||) cannot be applied to the atlas as
+    is: its ||b|| sums over all instances, so a small cable packed beside large
+    sheets would be solved far less tightly than alone.  The batch turns it
+    into the absolute test r.r <= rel^2 * min_i n_i * (alpha * 1 mV)^2, which
+    implies ||r_i|| <= rel ||b_i|| for every instance i whose RMS potential is
+    at least 1 mV (b_i = alpha V_i; a whole instance never sits at 0 mV).
 
 Instances share the model, dt, cell size, conductivity and the stimulus timing.
 A cable of n cells is the reference's initialize_grid_with_cable_mesh
@@ -152,6 +158,11 @@ class Batch:
         for bad in ("active", "stim_mask", "ionic_scale", "g_cell"):
             if bad in kw:
                 raise ValueError(f"{bad} is set by the batch layout")
+        rel = kw.pop("cg_rel_tol", 0.0)
+        if rel and rel > 0.0:
+            from . import alpha as alpha_of
+            a = alpha_of(kw.get("beta", 0.14), kw.get("cm", 1.0), kw.get("dt", 0.02), *h)
+            kw["cg_tol"] = rel * rel * min(inst.n for inst in self.instances) * (a * 1.0) ** 2
         self.tissue = Tissue(self.layout.shape, h=h, model=model, active=self.layout.active(),
                              stim_mask=self.layout.stim_mask(h[0]), ionic_scale=self.layout.ionic_scale(), **kw)
 

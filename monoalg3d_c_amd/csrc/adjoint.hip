@@ -237,7 +237,10 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
         return -2;
     }
     mono_tissue *t = nullptr;
-    if(int rc = mono_tissue_create(desc, &t)) return rc;
+    // one ODE step per PDE step: a dt_ode left at 0 means dt_pde here
+    mono_tissue_desc d1 = *desc;
+    if(!(d1.dt_ode > 0)) d1.dt_ode = d1.dt_pde;
+    if(int rc = mono_tissue_create(&d1, &t)) return rc;
     struct Guard {
         mono_tissue *t;
         ~Guard() { mono_tissue_destroy(t); }

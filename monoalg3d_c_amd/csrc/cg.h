@@ -39,6 +39,16 @@ struct CgConfig {
     int poll_chunk = 0;       // iterations enqueued between status polls (0 = adaptive)
 };
 
+// Device buffers of a CSR solve: r, p, q, z (n doubles each), two partial-sum
+// buffers (kMaxBlocks * kMaxRed doubles each), the device scalar record and a
+// host record the solve polls into (pinned or not).
+struct CsrBuffers {
+    double *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
+    double *P0 = nullptr, *P1 = nullptr;
+    CgScalars *sc = nullptr;
+    CgScalars *host_sc = nullptr;
+};
+
 struct CgStats {
     int iterations = 0;       // as the reference counts them (1 + CG iterations)
     double error = 0.0;       // final r.r (FP64)
@@ -46,9 +56,17 @@ struct CgStats {
     int inner_total = 0;      // mixed: total inner iterations
 };
 
+// CSR conjugate gradient (FP64) on caller-provided buffers.
+int csr_solve(const CsrMatrix &A, const double *b, double *x, const CgConfig &cfg, CgStats *st, hipStream_t s,
+              const CsrBuffers &w);
+// diag[i] = A(i, i) (0 where the row has no diagonal entry; the last entry wins)
+hipError_t launch_csr_diag(const CsrMatrix &A, double *diag, hipStream_t s);
+// b[i] = alpha[i] * vm[i] (UPDATE_MONODOMAIN)
+hipError_t launch_rhs(int64_t n, const double *alpha, const double *vm, double *b, hipStream_t s);
 class CgWorkspace {
   public:
-    int init(int64_t n, int64_t pitch_hint);
+    // fp32 = false: only the FP64 vectors (CSR and FP64 solves)
+    int init(int64_t n, int64_t pitch_hint, bool fp32 = true);
     int release();
     // Solve A x = b in FP64 (x is the warm start and the result).
     int solve_fp64(const OpView<double> &op, const double *b, double *x, const CgConfig &cfg, CgStats *st, hipStream_t s);

@@ -88,6 +88,15 @@ template <int NR> __device__ __forceinline__ void write_partials(double (&v)[NR]
     }
 }
 
+// The same into slot `slot` (a run of the plane-loop kernels).
+template <int NR> __device__ __forceinline__ void write_partials_at(double (&v)[NR], double *P, int slot) {
+    block_sum_all<NR>(v);
+    if(threadIdx.x == 0) {
+#pragma unroll
+        for(int k = 0; k < NR; k++) P[k * kMaxBlocks + slot] = v[k];
+    }
+}
+
 // Totals of the previous kernel's partials (fixed order; every thread gets them).
 template <int NR> __device__ __forceinline__ void sum_partials(const double *__restrict__ P, int nprev, double (&tot)[NR]) {
 #pragma unroll
@@ -247,9 +256,19 @@ constexpr int kNLD = (kPL + kBlock - 1) / kBlock;
 constexpr int kPLS = kNLD * kBlock;  // LDS plane stride: every lane of every load set has a slot
 constexpr int kRPT = kTX * kTY / kBlock;  // rows per thread
 
+// Work split of the plane-loop kernels.  The box is cut into nslab z slabs of
+// L planes and its xy plane into ntiles tiles; a run is (slab, group of tpr
+// consecutive tiles), numbered slab-major, so the runs of one slab are
+// neighbouring tiles.  Runs are a fixed function of the box and the device's
+// CU count: every run writes its own partial sums, and the next kernel adds
+// them in run order, so the reductions (and the solver's results) do not
+// depend on how many blocks the launch has.  Each block takes rpb consecutive
+// runs; blocks of one XCD take adjacent ones, so the tiles in flight on an XCD
+// at any time are neighbours at about the same depth and their shared halo
+// rows come from its L2.
 struct TileGeom {
     int ntx, nty, nblocks;
-    int64_t nitems, per;   // (tile, plane) items and items per block
+    int ntiles, L, nslab, tpr, nruns, rpb;
     int lo[3], hi[3];      // interior range of each axis (extent 1: its cell; extent 2: empty)
 };
 
@@ -498,18 +517,28 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
     }
 }
 
+// Call f(tile, z0, z1) for every tile and plane range of run r.
+template <class F> __device__ __forceinline__ void for_run(const TileGeom &g, int nz, int r, F &&f) {
+    const int slab = r / (g.nruns / g.nslab), grp = r % (g.nruns / g.nslab);
+    const int z0 = slab * g.L, z1 = z0 + g.L < nz ? z0 + g.L : nz;
+    const int t0 = grp * g.tpr, t1 = t0 + g.tpr < g.ntiles ? t0 + g.tpr : g.ntiles;
+    for(int t = t0; t < t1; t++) f(t, z0, z1);
+}
+
+// This block's runs: each run's partial sums go to its own slot of Pout.
 template <typename TV, typename TC, int CM, int PAT, bool FUSE, class Epi>
 __device__ __forceinline__ void tile_sweep(const OpView<TC> &op, const TileGeom &g, const TV *__restrict__ x, const TV *__restrict__ rz,
-                                           TV beta, TV *__restrict__ p_out, const Epi &epi, double (&v)[Epi::NR]) {
-    // runs of one XCD's blocks are adjacent (shared halos stay in its L2)
-    const int L = xcd_chunk(blockIdx.x, gridDim.x);
-    int64_t w = (int64_t)L * g.per;
-    const int64_t end = w + g.per < g.nitems ? w + g.per : g.nitems;
-    while(w < end) {
-        const int t = (int)(w / op.nz), z0 = (int)(w % op.nz);
-        const int z1 = (int64_t)op.nz - z0 < end - w ? op.nz : z0 + (int)(end - w);
-        tile_segment<TV, TC, CM, PAT, FUSE, Epi>(op, g, t, z0, z1, x, rz, beta, p_out, epi, v);
-        w += z1 - z0;
+                                           TV beta, TV *__restrict__ p_out, const Epi &epi, double *Pout) {
+    const int Lb = xcd_chunk(blockIdx.x, gridDim.x);
+    const int r0 = Lb * g.rpb, r1 = r0 + g.rpb < g.nruns ? r0 + g.rpb : g.nruns;
+    for(int r = r0; r < r1; r++) {
+        double v[Epi::NR];
+#pragma unroll
+        for(int j = 0; j < Epi::NR; j++) v[j] = 0.0;
+        for_run(g, op.nz, r, [&](int t, int z0, int z1) {
+            tile_segment<TV, TC, CM, PAT, FUSE, Epi>(op, g, t, z0, z1, x, rz, beta, p_out, epi, v);
+        });
+        write_partials_at<Epi::NR>(v, Pout, r);
     }
 }
 
@@ -518,23 +547,24 @@ __device__ __forceinline__ void tile_sweep(const OpView<TC> &op, const TileGeom 
 template <typename TV, typename TC>
 __device__ __forceinline__ void defer_apply(const OpView<TC> &op, const TileGeom &g, TV *__restrict__ x, const TV *__restrict__ p,
                                             TV alpha) {
-    const int L = xcd_chunk(blockIdx.x, gridDim.x);
-    const int64_t w0 = (int64_t)L * g.per;
-    const int64_t w1 = w0 + g.per < g.nitems ? w0 + g.per : g.nitems;
+    const int Lb = xcd_chunk(blockIdx.x, gridDim.x);
+    const int r0 = Lb * g.rpb, r1 = r0 + g.rpb < g.nruns ? r0 + g.rpb : g.nruns;
     const int64_t sz = (int64_t)op.nx * op.ny;
     const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
-    for(int64_t w = w0; w < w1; w++) {
-        const int t = (int)(w / op.nz), z = (int)(w % op.nz);
-        const int gx = (t % g.ntx) * kTX + tx, y0 = (t / g.ntx) * kTY;
-        if(gx >= op.nx) continue;
+    for(int r = r0; r < r1; r++)
+        for_run(g, op.nz, r, [&](int t, int z0, int z1) {
+            const int gx = (t % g.ntx) * kTX + tx, y0 = (t / g.ntx) * kTY;
+            if(gx >= op.nx) return;
+            for(int z = z0; z < z1; z++) {
 #pragma unroll
-        for(int q = 0; q < kRPT; q++) {
-            const int gy = y0 + ty + q * (kBlock / kTX);
-            if(gy >= op.ny) continue;
-            const int64_t c = gx + (int64_t)op.nx * gy + sz * z;
-            x[c] = x[c] + alpha * p[c];
-        }
-    }
+                for(int q = 0; q < kRPT; q++) {
+                    const int gy = y0 + ty + q * (kBlock / kTX);
+                    if(gy >= op.ny) continue;
+                    const int64_t c = gx + (int64_t)op.nx * gy + sz * z;
+                    x[c] = x[c] + alpha * p[c];
+                }
+            }
+        });
 }
 
 // Calibration (mode -2): every block records when it started and stays for
@@ -698,11 +728,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(OpView<TC> op, TileGeom g, c
         return;
     }
     const TV beta = (TV)betad;
-    double v[Epi::NR];
-#pragma unroll
-    for(int j = 0; j < Epi::NR; j++) v[j] = 0.0;
-    tile_sweep<TV, TC, CM, PAT, FUSE, Epi>(op, g, x, rz, beta, p_out, e, v);
-    write_partials<Epi::NR>(v, Pout);
+    tile_sweep<TV, TC, CM, PAT, FUSE, Epi>(op, g, x, rz, beta, p_out, e, Pout);
 }
 
 // q = A p, partial p.q
@@ -1017,10 +1043,14 @@ template <typename TC, class F> static void with_op(const OpView<TC> &op, F &&f)
     });
 }
 
-// Tiles over the whole box; (tile, plane) items split over one wave of resident
-// blocks with at least kMinRun planes per block.
+// Runs of the plane-loop kernels (see TileGeom), a function of the box and the
+// CU count only: about kRunsPerCU runs per CU (MONO_RUNS_PER_CU overrides),
+// slabs of at least kMinRun planes, at most kMaxBlocks runs (the partial-sum
+// slots).  The launch has one block per run, or as many blocks as are resident
+// at once when fewer (`slots`, measured), each then taking consecutive runs.
 constexpr int kMinRun = 8;
-template <typename TC> static TileGeom make_geom(const OpView<TC> &op, int slots) {
+constexpr int kRunsPerCU = 6;
+template <typename TC> static TileGeom make_geom(const OpView<TC> &op, int slots, int cus) {
     TileGeom g{};
     const int L[3] = {op.nx, op.ny, op.nz};
     for(int a = 0; a < 3; a++) {
@@ -1029,14 +1059,36 @@ template <typename TC> static TileGeom make_geom(const OpView<TC> &op, int slots
     }
     g.ntx = (op.nx + kTX - 1) / kTX;
     g.nty = (op.ny + kTY - 1) / kTY;
-    g.nitems = (int64_t)g.ntx * g.nty * op.nz;
-    int64_t nb = (g.nitems + kMinRun - 1) / kMinRun;
-    if(nb > slots) nb = slots;
-    if(nb > kMaxBlocks) nb = kMaxBlocks;
-    if(nb >= 16) nb = nb / 8 * 8;  // whole rounds of the XCD dispatch order
+    g.ntiles = g.ntx * g.nty;
+    static int per_cu = 0;
+    if(!per_cu) {
+        const char *e = getenv("MONO_RUNS_PER_CU");
+        per_cu = e && atoi(e) > 0 ? atoi(e) : kRunsPerCU;
+    }
+    const int64_t target = (int64_t)(cus > 0 ? cus : 1) * per_cu;
+    int64_t nslab = target / g.ntiles;
+    if(nslab < 1) nslab = 1;
+    int64_t len = (op.nz + nslab - 1) / nslab;
+    if(len < kMinRun) len = op.nz < kMinRun ? op.nz : kMinRun;
+    nslab = (op.nz + len - 1) / len;
+    g.L = (int)len;
+    g.nslab = (int)nslab;
+    // tiles per run: one, unless the partial-sum slots run out
+    int64_t groups = g.ntiles;
+    g.tpr = 1;
+    while(groups * nslab > kMaxBlocks) {
+        g.tpr++;
+        groups = (g.ntiles + g.tpr - 1) / g.tpr;
+    }
+    g.nruns = (int)(groups * nslab);
+    // blocks: one per run when that many are resident, in whole rounds of the
+    // XCD dispatch order
+    int64_t nb = g.nruns;
+    if(nb >= 16) nb = (nb + 7) / 8 * 8;
+    if(nb > slots) nb = slots >= 16 ? slots / 8 * 8 : slots;
     if(nb < 1) nb = 1;
+    g.rpb = (int)((g.nruns + nb - 1) / nb);
     g.nblocks = (int)nb;
-    g.per = (g.nitems + nb - 1) / nb;
     return g;
 }
 
@@ -1065,7 +1117,8 @@ static int grid_for(int64_t n) {
     return (int)(g < 1 ? 1 : g);
 }
 
-int CgWorkspace::init(int64_t n_, int64_t) {
+
+int CgWorkspace::init(int64_t n_, int64_t, bool fp32) {
     n = n_;
     const size_t b64 = sizeof(double) * (size_t)(n > 0 ? n : 1), b32 = sizeof(float) * (size_t)(n > 0 ? n : 1);
     double **v64[] = {&r, &p, &p2, &q, &z};
@@ -1075,6 +1128,7 @@ int CgWorkspace::init(int64_t n_, int64_t) {
     }
     float **v32[] = {&r32, &p32, &p32b, &q32, &d32};
     for(float **ptr : v32) {
+        if(!fp32) break;
         MONO_CHECK(hipMalloc(ptr, b32));
         MONO_CHECK(hipMemset(*ptr, 0, b32));
     }
@@ -1189,8 +1243,9 @@ static int poll(CgScalars *dev, CgScalars *host, hipStream_t s) {
     return 0;
 }
 
-// Launch one stencil kernel instance over its balanced geometry; returns its
-// block count (the number of partial sums it leaves), or -1 on a HIP error.
+// Launch one stencil kernel instance over its geometry; returns the number of
+// partial sums it leaves (its runs; the strip kernel: its blocks), or -1 on a
+// HIP error.
 // On first use per device the instance is calibrated: a launch in mode -2 with
 // more blocks than fit, each staying 30 us, shows how many are resident at
 // once (the occupancy query alone over-counts kernels with many scalar
@@ -1229,9 +1284,11 @@ static int launch_stencil(const OpView<TC> &op, const TV *x, const TV *rz, TV *p
             fprintf(stderr, "[mono] stencil<%s,%d,%d,%d,NR=%d>: %d blocks resident (occupancy query %d x %d CUs)\n",
                     sizeof(TV) == 8 ? "f64" : "f32", C, P, (int)FUSE, Epi::NR, slots[dev], per_cu, cus);
     }
-    const TileGeom g = make_geom(op, slots[dev]);
+    static int cus[64] = {};
+    if(!cus[dev]) MONO_CHECK(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
+    const TileGeom g = make_geom(op, slots[dev], cus[dev]);
     K<<<g.nblocks, kBlock, 0, s>>>(op, g, x, rz, p_out, epi, cs, mode, k, Pin, nprev, Pout, sc, so);
-    return g.nblocks;
+    return g.nruns;
 }
 
 int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x, const CgConfig &cfg, CgStats *st, hipStream_t s) {
@@ -1438,9 +1495,24 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
 }
 
 int CgWorkspace::solve_csr(const CsrMatrix &A, const double *b, double *x, const CgConfig &cfg, CgStats *st, hipStream_t s) {
+    CsrBuffers w;
+    w.r = r;
+    w.p = p;
+    w.q = q;
+    w.z = z;
+    w.P0 = P0;
+    w.P1 = P1;
+    w.sc = sc;
+    w.host_sc = host_sc;
+    return csr_solve(A, b, x, cfg, st, s, w);
+}
+
+int csr_solve(const CsrMatrix &A, const double *b, double *x, const CgConfig &cfg, CgStats *st, hipStream_t s, const CsrBuffers &w) {
     const int grid = grid_for(A.n);
     const unsigned sgrid = (unsigned)(((int64_t)A.n * 64 + kBlock - 1) / kBlock);
     const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, cfg.jacobi, cfg.max_outer, 0};
+    double *r = w.r, *p = w.p, *q = w.q, *z = w.z, *P0 = w.P0, *P1 = w.P1;
+    CgScalars *sc = w.sc, *host_sc = w.host_sc;
     k_csr_spmv<<<sgrid, kBlock, 0, s>>>(A.n, A.row_ptr, A.col, A.val, x, q, nullptr);
     k_csr_res<<<grid, kBlock, 0, s>>>(A.n, q, b, A.diag, r, p, z, cfg.jacobi, P0);
     MONO_CHECK(hipGetLastError());
@@ -1467,6 +1539,35 @@ int CgWorkspace::solve_csr(const CsrMatrix &A, const double *b, double *x, const
         st->inner_total = host_sc->iters - 1;
     }
     return 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_csr_diag(int n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                     const double *__restrict__ val, double *__restrict__ diag) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if(i >= n) return;
+    double d = 0.0;
+    for(int64_t k = rp[i]; k < rp[i + 1]; k++)
+        if(col[k] == (int32_t)i) d = val[k];
+    diag[i] = d;
+}
+
+hipError_t launch_csr_diag(const CsrMatrix &A, double *diag, hipStream_t s) {
+    const unsigned g = (unsigned)(((int64_t)A.n + kBlock - 1) / kBlock);
+    k_csr_diag<<<g, kBlock, 0, s>>>(A.n, A.row_ptr, A.col, A.val, diag);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_rhs(int64_t n, const double *__restrict__ alpha, const double *__restrict__ vm,
+                                                double *__restrict__ b) {
+    int64_t beg, end;
+    chunk_range(n, beg, end);
+    for(int64_t c = beg + threadIdx.x; c < end; c += kBlock) b[c] = vm[c] * alpha[c];
+}
+
+hipError_t launch_rhs(int64_t n, const double *alpha, const double *vm, double *b, hipStream_t s) {
+    if(n <= 0) return hipSuccess;
+    k_rhs<<<grid_for(n), kBlock, 0, s>>>(n, alpha, vm, b);
+    return hipGetLastError();
 }
 
 }  // namespace mono

@@ -17,58 +17,91 @@
 
 namespace mono {
 
-// FP64 path: the device's correctly rounded-or-nearly libm (OCML), so results
-// agree with the reference's glibc to rounding (divisions, compiled as a
-// Newton-refined reciprocal, are within 1 ulp).  FP32 path: the hardware
+// Every model function is host-and-device, so the same source also runs on the
+// host (tests compare it with the oracle there, without a GPU).
+#define MONO_HD __host__ __device__ __forceinline__
+
+// exp for the FP64 path: Cody & Waite's argument reduction (W. J. Cody and
+// W. Waite, Software Manual for the Elementary Functions, 1980) x = n ln2 + r,
+// |r| <= ln2/2, with ln2 split into its nearest double and the remainder
+// (both FMA-applied), exp(r) by a degree-11 polynomial: the Taylor series
+// through r^13 economized to degree 11 with Chebyshev polynomials on
+// [-0.34658, 0.34658] (Lanczos' economization, exact rational arithmetic,
+// coefficients then rounded; truncation below 4e-18 relative), scaled by 2^n.
+// Within 1 ulp of a correctly rounded exp for every finite argument the
+// models produce; no special cases: a non-finite argument gives NaN (a state
+// that has left the model's range is reported downstream as a NaN
+// potential).  The library exp spends a fifth of its instructions on those
+// range checks.
+MONO_HD double exp_cw(double x) {
+    const double n = rint(x * 0x1.71547652b82fep+0);  // log2(e)
+    double r = fma(-n, 0x1.62e42fefa39efp-1, x);     // ln2, nearest double
+    r = fma(-n, 0x1.abc9e3b39803fp-56, r);           // ln2 - that double
+    double p = 0x1.af780f134c082p-26;
+    p = fma(p, r, 0x1.28b3c5e683c14p-22);
+    p = fma(p, r, 0x1.71dde78948070p-19);
+    p = fma(p, r, 0x1.a01991d94f3c5p-16);
+    p = fma(p, r, 0x1.a01a01b7f87b9p-13);
+    p = fma(p, r, 0x1.6c16c187efa50p-10);
+    p = fma(p, r, 0x1.111111110db8cp-7);
+    p = fma(p, r, 0x1.555555554f0fcp-5);
+    p = fma(p, r, 0x1.5555555555562p-3);
+    p = fma(p, r, 0x1.0000000000011p-1);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)n);
+}
+
+// FP64 path: exp_cw above, the device's libm (OCML) for log/pow/expm1, so
+// results agree with the reference's glibc to rounding (divisions, compiled as
+// a Newton-refined reciprocal, are within 1 ulp).  FP32 path: the hardware
 // approximations (v_exp_f32 / v_log_f32 based, a few ulp), which halve the
 // kernel's instruction count; its accuracy is bounded by the activation-time
-// tests, not by bitwise agreement.
+// tests, not by bitwise agreement.  (Host compiles of the FP32 path use the
+// host libm.)
 #ifndef MONO_FP32_FAST_MATH
 #define MONO_FP32_FAST_MATH 1
 #endif
-template <typename T> __device__ __forceinline__ T dexp(T x);
-template <> __device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
-#if MONO_FP32_FAST_MATH
-template <> __device__ __forceinline__ float dexp<float>(float x) { return __expf(x); }
+#if MONO_FP32_FAST_MATH && defined(__HIP_DEVICE_COMPILE__)
+#define MONO_F32(fast, lib) fast
 #else
-template <> __device__ __forceinline__ float dexp<float>(float x) { return expf(x); }
+#define MONO_F32(fast, lib) lib
 #endif
-template <typename T> __device__ __forceinline__ T dexpm1(T x);
-template <> __device__ __forceinline__ double dexpm1<double>(double x) { return expm1(x); }
-template <> __device__ __forceinline__ float dexpm1<float>(float x) { return expm1f(x); }
-template <typename T> __device__ __forceinline__ T dlog(T x);
-template <> __device__ __forceinline__ double dlog<double>(double x) { return log(x); }
-#if MONO_FP32_FAST_MATH
-template <> __device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
-#else
-template <> __device__ __forceinline__ float dlog<float>(float x) { return logf(x); }
-#endif
-template <typename T> __device__ __forceinline__ T dpow(T x, T y);
-template <> __device__ __forceinline__ double dpow<double>(double x, double y) { return pow(x, y); }
-#if MONO_FP32_FAST_MATH
-template <> __device__ __forceinline__ float dpow<float>(float x, float y) { return __powf(x, y); }
-#else
-template <> __device__ __forceinline__ float dpow<float>(float x, float y) { return powf(x, y); }
-#endif
-template <typename T> __device__ __forceinline__ T dsqrt(T x);
-template <> __device__ __forceinline__ double dsqrt<double>(double x) { return sqrt(x); }
-template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
+template <typename T> MONO_HD T dexp(T x);
+template <> MONO_HD double dexp<double>(double x) { return exp_cw(x); }
+template <> MONO_HD float dexp<float>(float x) { return MONO_F32(__expf, expf)(x); }
+template <typename T> MONO_HD T dexpm1(T x);
+template <> MONO_HD double dexpm1<double>(double x) { return expm1(x); }
+template <> MONO_HD float dexpm1<float>(float x) { return expm1f(x); }
+template <typename T> MONO_HD T dlog(T x);
+template <> MONO_HD double dlog<double>(double x) { return log(x); }
+template <> MONO_HD float dlog<float>(float x) { return MONO_F32(__logf, logf)(x); }
+template <typename T> MONO_HD T dpow(T x, T y);
+template <> MONO_HD double dpow<double>(double x, double y) { return pow(x, y); }
+template <> MONO_HD float dpow<float>(float x, float y) { return MONO_F32(__powf, powf)(x, y); }
+template <typename T> MONO_HD T dsqrt(T x);
+template <> MONO_HD double dsqrt<double>(double x) { return sqrt(x); }
+template <> MONO_HD float dsqrt<float>(float x) { return sqrtf(x); }
 
 // x / (exp(x) - 1), finite at x = 0 (limit 1).  The FP32 path evaluates the
 // Goldman-Hodgkin-Katz flux factors with it: written as the reference writes
 // them, v F^2/RT * (...) / (exp(k v F/RT) - 1) is 0/0 whenever a cell's Vm
 // rounds to within ~1e-6 mV of 0 in float (millions of plateau cells cross 0 mV),
 // while in FP64 that window is ~1e-15 mV.
-template <typename T> __device__ __forceinline__ T x_over_expm1(T x) { return x == T(0) ? T(1) : x / dexpm1<T>(x); }
+template <typename T> MONO_HD T x_over_expm1(T x) { return x == T(0) ? T(1) : x / dexpm1<T>(x); }
 
 // Rush-Larsen update toward the steady state: inf + (x - inf) * exp(-dt / tau)
-template <typename T> __device__ __forceinline__ T rl_toward(T inf, T x, T dt, T tau) {
+template <typename T> MONO_HD T rl_toward(T inf, T x, T dt, T tau) {
     return inf + (x - inf) * dexp<T>(-dt / tau);
 }
 // the same update written as inf - (inf - x) * exp(-dt / tau) (ORd form)
-template <typename T> __device__ __forceinline__ T rl_from(T inf, T x, T dt, T tau) {
+template <typename T> MONO_HD T rl_from(T inf, T x, T dt, T tau) {
     return inf - (inf - x) * dexp<T>(-dt / tau);
 }
+// the ORd form with the decay factor exp(-dt / tau) given: gates that share a
+// time constant share one exponential, and a time constant written 1 / D
+// gives the factor as exp(-dt * D) with no division
+template <typename T> MONO_HD T rl_decay(T inf, T x, T decay) { return inf - (inf - x) * decay; }
 
 // Per-instance conductance scalings (batching / sweeps).  Identity = all 1.
 struct IonicScale {
@@ -82,7 +115,7 @@ struct IonicScale {
 // y[0] (V) is read-only here; the V increment is returned.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ T tt2006_step(T *y, T stim, T dt, T sKr, T sCaL, T sKs) {
+MONO_HD T tt2006_step(T *y, T stim, T dt, T sKr, T sCaL, T sKs) {
     const T V = y[0], Xr1 = y[1], Xr2 = y[2], Xs = y[3], m = y[4], h = y[5], j = y[6];
     const T d = y[7], f = y[8], f2 = y[9], fCass = y[10], s = y[11], r = y[12];
     const T Ca_i = y[13], Ca_SR = y[14], Ca_ss = y[15], Rp = y[16], Na_i = y[17], K_i = y[18];
@@ -251,7 +284,7 @@ __device__ __forceinline__ T tt2006_step(T *y, T stim, T dt, T sKr, T sCaL, T sK
 // (V, h); the stimulus is added.  scale multiplies tau_close.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ T ms2003_step(T *y, T stim, T dt, T s_tau_close) {
+MONO_HD T ms2003_step(T *y, T stim, T dt, T s_tau_close) {
     const T V = y[0], h = y[1];
     const T tau_in = T(0.3), tau_out = T(6.0), V_gate = T(0.13), tau_open = T(120.0);
     const T tau_close = T(150.0) * s_tau_close;
@@ -267,14 +300,14 @@ __device__ __forceinline__ T ms2003_step(T *y, T stim, T dt, T s_tau_close) {
 
 namespace mono {
 
-template <typename T> __device__ __forceinline__ T cube(T x) { return x * x * x; }
+template <typename T> MONO_HD T cube(T x) { return x * x * x; }
 
 // State sink of a model step: st(s, value) is called once per state s >= 1 as
 // soon as its new value is final.  A kernel that passes a sink writing state
 // rows straight to HBM stores each gate right after its update, so the gate is
 // dead for the rest of the step instead of held in registers until the end.
 struct NullStore {
-    template <typename T> __device__ __forceinline__ void operator()(int, T) const {}
+    template <typename T> MONO_HD void operator()(int, T) const {}
 };
 
 // ---------------------------------------------------------------------------
@@ -286,7 +319,7 @@ struct NullStore {
 // The new states are written to y[1..40] and also passed to `st`.
 // ---------------------------------------------------------------------------
 template <typename T, typename St = NullStore>
-__device__ __forceinline__ T ord2011_step(T *y, T stim, T dt, St st = St()) {
+MONO_HD T ord2011_step(T *y, T stim, T dt, St st = St()) {
     const T v = y[0];
     T nai = y[1], nass = y[2], ki = y[3], kss = y[4], cai = y[5], cass = y[6];
     T cansr = y[7], cajsr = y[8], m = y[9], hf = y[10], hs = y[11], j = y[12], hsp = y[13];
@@ -324,44 +357,52 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt, St st = St()) {
     const T fCaMK = (T(1.0f) / (T(1.0f) + KmCaMK / CaMKa));  // fINap, fINaLp, fItop, fICaLp
 
     // INa
+    // Time constants written 1/D enter the Rush-Larsen factor as exp(-dt D)
+    // and those written c + 1/D as exp(-dt D / (c D + 1)), one division
+    // instead of two; gates with the same time constant share the factor, and
+    // exp((v + a) / k) of a second offset a' is the first times exp((a' - a) / k)
+    // (a constant).  Each differs from the reference's expression by a few
+    // ulp of the exponent or the result.
     const T mss = T(1.0f) / (T(1.0f) + dexp<T>((-(v + T(39.57f))) / T(9.871f)));
-    const T tm = T(1.0f) / (T(6.765f) * dexp<T>((v + T(11.64f)) / T(34.77f)) + T(8.552f) * dexp<T>(-(v + T(77.42f)) / T(5.955f)));
-    m = rl_from<T>(mss, m, dt, tm);
+    const T Dm = T(6.765f) * dexp<T>((v + T(11.64f)) / T(34.77f)) + T(8.552f) * dexp<T>(-(v + T(77.42f)) / T(5.955f));
+    const T Em = dexp<T>(-dt * Dm);  // m and mL share tm = 1 / Dm
+    m = rl_decay<T>(mss, m, Em);
     st(9, m);
-    const T hss = T(1.0f) / (T(1) + dexp<T>((v + T(82.90f)) / T(6.086f)));
-    const T thf = T(1.0f) / (T(1.432e-5f) * dexp<T>(-(v + T(1.196f)) / T(6.285f)) + T(6.149f) * dexp<T>((v + T(0.5096f)) / T(20.27f)));
-    const T ths = T(1.0f) / (T(0.009794f) * dexp<T>(-(v + T(17.95f)) / T(28.05f)) + T(0.3343f) * dexp<T>((v + T(5.730f)) / T(56.66f)));
+    const T eh = dexp<T>((v + T(82.90f)) / T(6.086f));
+    const T hss = T(1.0f) / (T(1) + eh);
+    const T Dhf = T(1.432e-5f) * dexp<T>(-(v + T(1.196f)) / T(6.285f)) + T(6.149f) * dexp<T>((v + T(0.5096f)) / T(20.27f));
+    const T Dhs = T(0.009794f) * dexp<T>(-(v + T(17.95f)) / T(28.05f)) + T(0.3343f) * dexp<T>((v + T(5.730f)) / T(56.66f));
     const T Ahf = T(0.99);
     const T Ahs = T(1.0f) - Ahf;
-    hf = rl_from<T>(hss, hf, dt, thf);
+    hf = rl_decay<T>(hss, hf, dexp<T>(-dt * Dhf));
     st(10, hf);
-    hs = rl_from<T>(hss, hs, dt, ths);
+    hs = rl_decay<T>(hss, hs, dexp<T>(-dt * Dhs));
     st(11, hs);
     const T h = Ahf * hf + Ahs * hs;
     const T jss = hss;
-    const T tj = T(2.038f) + T(1.0f) / (T(0.02136f) * dexp<T>(-(v + T(100.6f)) / T(8.281f)) + T(0.3052f) * dexp<T>((v + T(0.9941f)) / T(38.45f)));
-    j = rl_from<T>(jss, j, dt, tj);
+    const T Dj = T(0.02136f) * dexp<T>(-(v + T(100.6f)) / T(8.281f)) + T(0.3052f) * dexp<T>((v + T(0.9941f)) / T(38.45f));
+    const T rtj = Dj / (T(2.038f) * Dj + T(1.0f));  // 1 / tj, tj = 2.038 + 1 / Dj
+    j = rl_decay<T>(jss, j, dexp<T>(-dt * rtj));
     st(12, j);
-    const T hssp = T(1.0f) / (T(1.0f) + dexp<T>((v + T(89.1f)) / T(6.086f)));
-    const T thsp = T(3.0f) * ths;
-    hsp = rl_from<T>(hssp, hsp, dt, thsp);
+    const T hssp = T(1.0f) / (T(1.0f) + eh * dexp<T>((T(89.1f) - T(82.90f)) / T(6.086f)));
+    hsp = rl_decay<T>(hssp, hsp, dexp<T>(-dt * Dhs / T(3.0f)));  // thsp = 3 ths
     st(13, hsp);
     const T hp = Ahf * hf + Ahs * hsp;
-    const T tjp = T(1.46f) * tj;
-    jp = rl_from<T>(jss, jp, dt, tjp);
+    jp = rl_decay<T>(jss, jp, dexp<T>(-dt * rtj / T(1.46f)));  // tjp = 1.46 tj
     st(14, jp);
     const T GNa = T(75);
     const T INa = GNa * (v - ENa) * m * m * m * ((T(1.0f) - fCaMK) * h * j + fCaMK * hp * jp);
 
     // INaL
     const T mLss = T(1.0f) / (T(1.0f) + dexp<T>((-(v + T(42.85f))) / T(5.264f)));
-    mL = rl_from<T>(mLss, mL, dt, tm);
+    mL = rl_decay<T>(mLss, mL, Em);
     st(15, mL);
-    const T hLss = T(1.0f) / (T(1.0f) + dexp<T>((v + T(87.61f)) / T(7.488f)));
+    const T ehL = dexp<T>((v + T(87.61f)) / T(7.488f));
+    const T hLss = T(1.0f) / (T(1.0f) + ehL);
     const T thL = T(200.0);
     hL = rl_from<T>(hLss, hL, dt, thL);
     st(16, hL);
-    const T hLssp = T(1.0f) / (T(1.0f) + dexp<T>((v + T(93.81f)) / T(7.488f)));
+    const T hLssp = T(1.0f) / (T(1.0f) + ehL * dexp<T>((T(93.81f) - T(87.61f)) / T(7.488f)));
     const T thLp = T(3.0f) * thL;
     hLp = rl_from<T>(hLssp, hLp, dt, thLp);
     st(17, hLp);
@@ -369,31 +410,37 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt, St st = St()) {
     const T INaL = GNaL * (v - ENa) * mL * ((T(1.0f) - fCaMK) * hL + fCaMK * hLp);
 
     // Ito
-    const T ass = T(1.0f) / (T(1.0f) + dexp<T>((-(v - T(14.34f))) / T(14.82f)));
-    const T ta = T(1.0515f) / (T(1.0f) / (T(1.2089f) * (T(1.0f) + dexp<T>(-(v - T(18.4099f)) / T(29.3814f)))) +
-                               T(3.5f) / (T(1.0f) + dexp<T>((v + T(100.0f)) / T(29.3814f))));
-    a = rl_from<T>(ass, a, dt, ta);
+    const T ea = dexp<T>((-(v - T(14.34f))) / T(14.82f));
+    const T ass = T(1.0f) / (T(1.0f) + ea);
+    // ta = 1.0515 / Da: a and ap share the factor
+    const T Da = T(1.0f) / (T(1.2089f) * (T(1.0f) + dexp<T>(-(v - T(18.4099f)) / T(29.3814f)))) +
+                 T(3.5f) / (T(1.0f) + dexp<T>((v + T(100.0f)) / T(29.3814f)));
+    const T Ea = dexp<T>(-dt * Da / T(1.0515f));
+    a = rl_decay<T>(ass, a, Ea);
     st(18, a);
     const T iss = T(1.0f) / (T(1.0f) + dexp<T>((v + T(43.94f)) / T(5.711f)));
-    const T tiF = T(4.562f) + T(1.0f) / (T(0.3933f) * dexp<T>((-(v + T(100.0f))) / T(100.0f)) + T(0.08004f) * dexp<T>((v + T(50.0f)) / T(16.59f)));
-    const T tiS = T(23.62f) + T(1.0f) / (T(0.001416f) * dexp<T>((-(v + T(96.52f))) / T(59.05f)) + T(1.780e-8f) * dexp<T>((v + T(114.1f)) / T(8.079f)));
+    const T DiF = T(0.3933f) * dexp<T>((-(v + T(100.0f))) / T(100.0f)) + T(0.08004f) * dexp<T>((v + T(50.0f)) / T(16.59f));
+    const T DiS = T(0.001416f) * dexp<T>((-(v + T(96.52f))) / T(59.05f)) + T(1.780e-8f) * dexp<T>((v + T(114.1f)) / T(8.079f));
+    const T rtiF = DiF / (T(4.562f) * DiF + T(1.0f));   // 1 / tiF
+    const T rtiS = DiS / (T(23.62f) * DiS + T(1.0f));   // 1 / tiS
     const T AiF = T(1.0f) / (T(1.0f) + dexp<T>((v - T(213.6f)) / T(151.2f)));
     const T AiS = T(1.0f) - AiF;
-    iF = rl_from<T>(iss, iF, dt, tiF);
+    iF = rl_decay<T>(iss, iF, dexp<T>(-dt * rtiF));
     st(19, iF);
-    iS = rl_from<T>(iss, iS, dt, tiS);
+    iS = rl_decay<T>(iss, iS, dexp<T>(-dt * rtiS));
     st(20, iS);
     const T ii = AiF * iF + AiS * iS;
-    const T assp = T(1.0f) / (T(1.0f) + dexp<T>((-(v - T(24.34f))) / T(14.82f)));
-    ap = rl_from<T>(assp, ap, dt, ta);
+    const T assp = T(1.0f) / (T(1.0f) + ea * dexp<T>((T(24.34f) - T(14.34f)) / T(14.82f)));
+    ap = rl_decay<T>(assp, ap, Ea);
     st(21, ap);
     const T dti_develop = T(1.354f) + T(1.0e-4f) / (dexp<T>((v - T(167.4f)) / T(15.89f)) + dexp<T>(-(v - T(12.23f)) / T(0.2154f)));
-    const T dti_recover = T(1.0f) - T(0.5f) / (T(1.0f) + dexp<T>((v + T(70.0f)) / T(20.0f)));
-    const T tiFp = dti_develop * dti_recover * tiF;
-    const T tiSp = dti_develop * dti_recover * tiS;
-    iFp = rl_from<T>(iss, iFp, dt, tiFp);
+    const T e20 = dexp<T>((v + T(70.0f)) / T(20.0f));
+    const T dti_recover = T(1.0f) - T(0.5f) / (T(1.0f) + e20);
+    // tiFp = dti tiF, tiSp = dti tiS with dti = dti_develop * dti_recover
+    const T rdti = T(1.0f) / (dti_develop * dti_recover);
+    iFp = rl_decay<T>(iss, iFp, dexp<T>(-dt * rtiF * rdti));
     st(22, iFp);
-    iSp = rl_from<T>(iss, iSp, dt, tiSp);
+    iSp = rl_decay<T>(iss, iSp, dexp<T>(-dt * rtiS * rdti));
     st(23, iSp);
     const T ip = AiF * iFp + AiS * iSp;
     const T Gto = T(0.02);
@@ -401,38 +448,42 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt, St st = St()) {
 
     // ICaL, ICaNa, ICaK
     const T dss = T(1.0f) / (T(1.0f) + dexp<T>((-(v + T(3.940f))) / T(4.230f)));
-    const T td = T(0.6f) + T(1.0f) / (dexp<T>(T(-0.05f) * (v + T(6.0f))) + dexp<T>(T(0.09f) * (v + T(14.0f))));
-    d = rl_from<T>(dss, d, dt, td);
+    const T Dd = dexp<T>(T(-0.05f) * (v + T(6.0f))) + dexp<T>(T(0.09f) * (v + T(14.0f)));
+    d = rl_decay<T>(dss, d, dexp<T>(-dt * (Dd / (T(0.6f) * Dd + T(1.0f)))));  // td = 0.6 + 1 / Dd
     st(24, d);
     const T fss = T(1.0f) / (T(1.0f) + dexp<T>((v + T(19.58f)) / T(3.696f)));
-    const T tff = T(7.0f) + T(1.0f) / (T(0.0045f) * dexp<T>(-(v + T(20.0f)) / T(10.0f)) + T(0.0045f) * dexp<T>((v + T(20.0f)) / T(10.0f)));
-    const T tfs = T(1000.0f) + T(1.0f) / (T(0.000035f) * dexp<T>(-(v + T(5.0f)) / T(4.0f)) + T(0.000035f) * dexp<T>((v + T(5.0f)) / T(6.0f)));
+    // exp((v + 20) / 10) and exp(-(v + 20) / 10) are reciprocals; Afcaf's
+    // exp((v - 10) / 10) is the first times exp(-3)
+    const T e10 = dexp<T>((v + T(20.0f)) / T(10.0f));
+    const T Dff = T(0.0045f) * (T(1.0f) / e10) + T(0.0045f) * e10;
+    const T Dfs = T(0.000035f) * dexp<T>(-(v + T(5.0f)) / T(4.0f)) + T(0.000035f) * dexp<T>((v + T(5.0f)) / T(6.0f));
+    const T rtff = Dff / (T(7.0f) * Dff + T(1.0f));       // 1 / tff
     const T Aff = T(0.6);
     const T Afs = T(1.0f) - Aff;
-    ff = rl_from<T>(fss, ff, dt, tff);
+    ff = rl_decay<T>(fss, ff, dexp<T>(-dt * rtff));
     st(25, ff);
-    fs = rl_from<T>(fss, fs, dt, tfs);
+    fs = rl_decay<T>(fss, fs, dexp<T>(-dt * (Dfs / (T(1000.0f) * Dfs + T(1.0f)))));
     st(26, fs);
     const T f = Aff * ff + Afs * fs;
     const T fcass = fss;
-    const T tfcaf = T(7.0f) + T(1.0f) / (T(0.04f) * dexp<T>(-(v - T(4.0f)) / T(7.0f)) + T(0.04f) * dexp<T>((v - T(4.0f)) / T(7.0f)));
-    const T tfcas = T(100.0f) + T(1.0f) / (T(0.00012f) * dexp<T>(-v / T(3.0f)) + T(0.00012f) * dexp<T>(v / T(7.0f)));
-    const T Afcaf = T(0.3f) + T(0.6f) / (T(1.0f) + dexp<T>((v - T(10.0f)) / T(10.0f)));
+    const T e7 = dexp<T>((v - T(4.0f)) / T(7.0f));
+    const T Dfcaf = T(0.04f) * (T(1.0f) / e7) + T(0.04f) * e7;
+    const T Dfcas = T(0.00012f) * dexp<T>(-v / T(3.0f)) + T(0.00012f) * dexp<T>(v / T(7.0f));
+    const T rtfcaf = Dfcaf / (T(7.0f) * Dfcaf + T(1.0f));  // 1 / tfcaf
+    const T Afcaf = T(0.3f) + T(0.6f) / (T(1.0f) + e10 * dexp<T>((T(-10.0f) - T(20.0f)) / T(10.0f)));
     const T Afcas = T(1.0f) - Afcaf;
-    fcaf = rl_from<T>(fcass, fcaf, dt, tfcaf);
+    fcaf = rl_decay<T>(fcass, fcaf, dexp<T>(-dt * rtfcaf));
     st(27, fcaf);
-    fcas = rl_from<T>(fcass, fcas, dt, tfcas);
+    fcas = rl_decay<T>(fcass, fcas, dexp<T>(-dt * (Dfcas / (T(100.0f) * Dfcas + T(1.0f)))));
     st(28, fcas);
     const T fca = Afcaf * fcaf + Afcas * fcas;
     const T tjca = T(75.0);
     jca = rl_from<T>(fcass, jca, dt, tjca);
     st(29, jca);
-    const T tffp = T(2.5f) * tff;
-    ffp = rl_from<T>(fss, ffp, dt, tffp);
+    ffp = rl_decay<T>(fss, ffp, dexp<T>(-dt * rtff / T(2.5f)));  // tffp = 2.5 tff
     st(31, ffp);
     const T fp = Aff * ffp + Afs * fs;
-    const T tfcafp = T(2.5f) * tfcaf;
-    fcafp = rl_from<T>(fcass, fcafp, dt, tfcafp);
+    fcafp = rl_decay<T>(fcass, fcafp, dexp<T>(-dt * rtfcaf / T(2.5f)));  // tfcafp = 2.5 tfcaf
     st(32, fcafp);
     const T fcap = Afcaf * fcafp + Afcas * fcas;
     const T Kmn = T(0.002);
@@ -443,8 +494,8 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt, St st = St()) {
     const T anca = T(1.0f) / (k2n / km2n + kq2 * kq2);
     nca = anca * k2n / km2n - (anca * k2n / km2n - nca) * dexp<T>(-km2n * dt);
     st(30, nca);
-    const T e2 = dexp<T>(T(2.0f) * vfrt);
-    const T e1 = dexp<T>(T(1.0f) * vfrt);
+    const T e1 = dexp<T>(vfrt);
+    const T e2 = e1 * e1;  // exp(2 vfrt)
     T PhiCaL, PhiCaNa, PhiCaK;
     if constexpr(sizeof(T) == 8) {
         PhiCaL = T(4.0f) * vffrt * (cass * e2 - T(0.341f) * cao) / (e2 - T(1.0f));
@@ -472,27 +523,28 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt, St st = St()) {
 
     // IKr
     const T xrss = T(1.0f) / (T(1.0f) + dexp<T>((-(v + T(8.337f))) / T(6.789f)));
-    const T txrf = T(12.98f) + T(1.0f) / (T(0.3652f) * dexp<T>((v - T(31.66f)) / T(3.869f)) + T(4.123e-5f) * dexp<T>((-(v - T(47.78f))) / T(20.38f)));
-    const T txrs = T(1.865f) + T(1.0f) / (T(0.06629f) * dexp<T>((v - T(34.70f)) / T(7.355f)) + T(1.128e-5f) * dexp<T>((-(v - T(29.74f))) / T(25.94f)));
+    const T Dxrf = T(0.3652f) * dexp<T>((v - T(31.66f)) / T(3.869f)) + T(4.123e-5f) * dexp<T>((-(v - T(47.78f))) / T(20.38f));
+    const T Dxrs = T(0.06629f) * dexp<T>((v - T(34.70f)) / T(7.355f)) + T(1.128e-5f) * dexp<T>((-(v - T(29.74f))) / T(25.94f));
     const T Axrf = T(1.0f) / (T(1.0f) + dexp<T>((v + T(54.81f)) / T(38.21f)));
     const T Axrs = T(1.0f) - Axrf;
-    xrf = rl_from<T>(xrss, xrf, dt, txrf);
+    xrf = rl_decay<T>(xrss, xrf, dexp<T>(-dt * (Dxrf / (T(12.98f) * Dxrf + T(1.0f)))));
     st(33, xrf);
-    xrs = rl_from<T>(xrss, xrs, dt, txrs);
+    xrs = rl_decay<T>(xrss, xrs, dexp<T>(-dt * (Dxrs / (T(1.865f) * Dxrs + T(1.0f)))));
     st(34, xrs);
     const T xr = Axrf * xrf + Axrs * xrs;
-    const T rkr = T(1.0f) / (T(1.0f) + dexp<T>((v + T(55.0f)) / T(75.0f))) * T(1.0f) / (T(1.0f) + dexp<T>((v - T(10.0f)) / T(30.0f)));
+    const T rkr = T(1.0f) / ((T(1.0f) + dexp<T>((v + T(55.0f)) / T(75.0f))) * (T(1.0f) + dexp<T>((v - T(10.0f)) / T(30.0f))));
     const T GKr = T(0.046);
     const T IKr = GKr * dsqrt<T>(ko / T(5.4f)) * xr * rkr * (v - EK);
 
     // IKs
     const T xs1ss = T(1.0f) / (T(1.0f) + dexp<T>((-(v + T(11.60f))) / T(8.932f)));
-    const T txs1 = T(817.3f) + T(1.0f) / (T(2.326e-4f) * dexp<T>((v + T(48.28f)) / T(17.80f)) + T(0.001292f) * dexp<T>((-(v + T(210.0f))) / T(230.0f)));
-    xs1 = rl_from<T>(xs1ss, xs1, dt, txs1);
+    const T Dxs1 = T(2.326e-4f) * dexp<T>((v + T(48.28f)) / T(17.80f)) + T(0.001292f) * dexp<T>((-(v + T(210.0f))) / T(230.0f));
+    xs1 = rl_decay<T>(xs1ss, xs1, dexp<T>(-dt * (Dxs1 / (T(817.3f) * Dxs1 + T(1.0f)))));  // txs1 = 817.3 + 1 / Dxs1
     st(35, xs1);
     const T xs2ss = xs1ss;
-    const T txs2 = T(1.0f) / (T(0.01f) * dexp<T>((v - T(50.0f)) / T(20.0f)) + T(0.0193f) * dexp<T>((-(v + T(66.54f))) / T(31.0f)));
-    xs2 = rl_from<T>(xs2ss, xs2, dt, txs2);
+    // txs2 = 1 / Dxs2; exp((v - 50) / 20) is dti_recover's exp((v + 70) / 20) times exp(-6)
+    const T Dxs2 = T(0.01f) * (e20 * dexp<T>((T(-50.0f) - T(70.0f)) / T(20.0f))) + T(0.0193f) * dexp<T>((-(v + T(66.54f))) / T(31.0f));
+    xs2 = rl_decay<T>(xs2ss, xs2, dexp<T>(-dt * Dxs2));
     st(36, xs2);
     const T KsCa = T(1.0f) + T(0.6f) / (T(1.0f) + dpow<T>(T(3.8e-5f) / cai, T(1.4f)));
     const T GKs = T(0.0034);
@@ -500,8 +552,8 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt, St st = St()) {
 
     // IK1
     const T xk1ss = T(1.0f) / (T(1.0f) + dexp<T>(-(v + T(2.5538f) * ko + T(144.59f)) / (T(1.5692f) * ko + T(3.8115f))));
-    const T txk1 = T(122.2f) / (dexp<T>((-(v + T(127.2f))) / T(20.36f)) + dexp<T>((v + T(236.8f)) / T(69.33f)));
-    xk1 = rl_from<T>(xk1ss, xk1, dt, txk1);
+    const T Dxk1 = dexp<T>((-(v + T(127.2f))) / T(20.36f)) + dexp<T>((v + T(236.8f)) / T(69.33f));
+    xk1 = rl_decay<T>(xk1ss, xk1, dexp<T>(-dt * Dxk1 / T(122.2f)));  // txk1 = 122.2 / Dxk1
     st(37, xk1);
     const T rk1 = T(1.0f) / (T(1.0f) + dexp<T>((v + T(105.8f) - T(2.6f) * ko) / T(9.493f)));
     const T GK1 = T(0.1908);
@@ -548,8 +600,8 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt, St st = St()) {
         const T x2 = k1 * k7 * (k4 + k5) + k4 * k6 * (k1 + k8);
         const T x3 = k1 * k3 * (k7 + k6) + k8 * k6 * (k2 + k3);
         const T x4 = k2 * k8 * (k4 + k5) + k3 * k5 * (k1 + k8);
-        const T xs = x1 + x2 + x3 + x4;
-        const T E1 = x1 / xs, E2 = x2 / xs, E3 = x3 / xs, E4 = x4 / xs;
+        const T rxs = T(1.0f) / (x1 + x2 + x3 + x4);
+        const T E1 = x1 * rxs, E2 = x2 * rxs, E3 = x3 * rxs, E4 = x4 * rxs;
         const T cr = KmCaAct / ca;
         const T allo = T(1.0f) / (T(1.0f) + cr * cr);
         const T JncxNa = T(3.0f) * (E4 * k7 - E1 * k8) + E3 * k4pp - E2 * k3pp;
@@ -585,8 +637,8 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt, St st = St()) {
     const T x2 = b2 * b1 * b4 + a1 * a2 * a3 + a3 * b1 * b4 + a2 * a3 * b4;
     const T x3 = a2 * a3 * a4 + b3 * b2 * b1 + b2 * b1 * a4 + a3 * a4 * b1;
     const T x4 = b4 * b3 * b2 + a3 * a4 * a1 + b2 * a4 * a1 + b3 * b2 * a1;
-    const T xs = x1 + x2 + x3 + x4;
-    const T E1 = x1 / xs, E2 = x2 / xs, E3 = x3 / xs, E4 = x4 / xs;
+    const T rxs = T(1.0f) / (x1 + x2 + x3 + x4);
+    const T E1 = x1 * rxs, E2 = x2 * rxs, E3 = x3 * rxs, E4 = x4 * rxs;
     const T zk = T(1.0f);
     const T JnakNa = T(3.0f) * (E1 * a3 - E2 * b3);
     const T JnakK = T(2.0f) * (E4 * b1 - E3 * a1);
@@ -598,7 +650,7 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt, St st = St()) {
     const T GKb = T(0.003);
     const T IKb = GKb * xkb * (v - EK);
     const T PNab = T(3.75e-10);
-    const T ev = dexp<T>(vfrt);
+    const T ev = e1;
     const T PCab = T(2.5e-8);
     T INab, ICab;
     if constexpr(sizeof(T) == 8) {
@@ -650,15 +702,18 @@ __device__ __forceinline__ T ord2011_step(T *y, T stim, T dt, St st = St()) {
     nass += dt * (-(ICaNa + T(3.0) * INaCa_ss) * Acap / (F * vss) - JdiffNa);
     ki += dt * (-(Ito + IKr + IKs + IK1 + IKb + stim - T(2.0f) * INaK) * Acap / (F * vmyo) + JdiffK * vss / vmyo);
     kss += dt * (-(ICaK)*Acap / (F * vss) - JdiffK);
+    // buffering factors 1 / (1 + a / c1^2 + b / c2^2) over one division
     const T c1 = kmcmdn + cai, c2 = kmtrpn + cai;
-    const T Bcai = T(1.0f) / (T(1.0f) + cmdnmax * kmcmdn / (c1 * c1) + trpnmax * kmtrpn / (c2 * c2));
+    const T c1s = c1 * c1, c2s = c2 * c2;
+    const T Bcai = (c1s * c2s) / (c1s * c2s + cmdnmax * kmcmdn * c2s + trpnmax * kmtrpn * c1s);
     cai += dt * (Bcai * (-(IpCa + ICab - T(2.0f) * INaCa_i) * Acap / (T(2.0f) * F * vmyo) - Jup * vnsr / vmyo + Jdiff * vss / vmyo));
     const T c3 = KmBSR + cass, c4 = KmBSL + cass;
-    const T Bcass = T(1.0f) / (T(1.0f) + BSRmax * KmBSR / (c3 * c3) + BSLmax * KmBSL / (c4 * c4));
+    const T c3s = c3 * c3, c4s = c4 * c4;
+    const T Bcass = (c3s * c4s) / (c3s * c4s + BSRmax * KmBSR * c4s + BSLmax * KmBSL * c3s);
     cass += dt * (Bcass * (-(ICaL - T(2.0f) * INaCa_ss) * Acap / (T(2.0f) * F * vss) + Jrel * vjsr / vss - Jdiff));
     cansr += dt * (Jup - Jtr * vjsr / vnsr);
-    const T c5 = kmcsqn + cajsr;
-    const T Bcajsr = T(1.0f) / (T(1.0f) + csqnmax * kmcsqn / (c5 * c5));
+    const T c5s = (kmcsqn + cajsr) * (kmcsqn + cajsr);
+    const T Bcajsr = c5s / (c5s + csqnmax * kmcsqn);
     cajsr += dt * (Bcajsr * (Jtr - Jrel));
     st(1, nai); st(2, nass); st(3, ki); st(4, kss); st(5, cai); st(6, cass); st(7, cansr); st(8, cajsr);
     st(40, CaMKt);

@@ -22,4 +22,22 @@ struct Observer {
     int observe(const double *v, const uint8_t *active, double t, hipStream_t s);
 };
 
+// First action potential of every cell for precision studies (no reference
+// counterpart): activation time (upward crossing of a threshold, linearly
+// interpolated between samples), peak and APD90 (time from activation to the
+// downward crossing of V90 = Vmax - 0.9 (Vmax - Vrest) after the peak, also
+// interpolated; Vrest = the cell's potential at set-up), sampled on the FP64
+// potential at every call.
+struct ApdTracker {
+    int64_t n = 0;
+    double thr = -30.0;
+    double *prev = nullptr, *vrest = nullptr, *lat = nullptr, *vmax = nullptr, *apd90 = nullptr;
+    double tprev = 0.0;
+    int8_t *phase = nullptr;  // 0 resting, 1 activated, 2 repolarised
+
+    int init(int64_t n, double thr, const double *v, double t, hipStream_t s);
+    int release();
+    int sample(const double *v, const uint8_t *active, double t, hipStream_t s);
+};
+
 }  // namespace mono

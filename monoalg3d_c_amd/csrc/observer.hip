@@ -122,4 +122,85 @@ int Observer::observe(const double *v, const uint8_t *active, double t, hipStrea
     return 0;
 }
 
+// One sample of the APD90 tracker at time t (previous sample at tp).
+__global__ __launch_bounds__(256) void k_apd_sample(int64_t n, const double *__restrict__ v, const uint8_t *__restrict__ active,
+                                                    double t, double tp, double thr, double *__restrict__ prev,
+                                                    const double *__restrict__ vrest, double *__restrict__ lat,
+                                                    double *__restrict__ vmax, double *__restrict__ apd90, int8_t *__restrict__ phase) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    if(active && !active[i]) return;
+    const double vi = v[i], vp = prev[i];
+    int ph = phase[i];
+    if(ph == 0 && vp < thr && vi >= thr) {
+        lat[i] = tp + (t - tp) * ((thr - vp) / (vi - vp));
+        vmax[i] = vi;
+        ph = 1;
+    } else if(ph == 1) {
+        if(vi > vmax[i]) {
+            vmax[i] = vi;
+        } else {
+            const double v90 = vmax[i] - 0.9 * (vmax[i] - vrest[i]);
+            if(vp > v90 && vi <= v90) {
+                apd90[i] = tp + (t - tp) * ((v90 - vp) / (vi - vp)) - lat[i];
+                ph = 2;
+            }
+        }
+    }
+    prev[i] = vi;
+    phase[i] = (int8_t)ph;
+}
+
+__global__ __launch_bounds__(256) void k_apd_init(int64_t n, const double *__restrict__ v, double *__restrict__ prev,
+                                                  double *__restrict__ vrest, double *__restrict__ lat, double *__restrict__ vmax,
+                                                  double *__restrict__ apd90, int8_t *__restrict__ phase) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    prev[i] = v[i];
+    vrest[i] = v[i];
+    lat[i] = NAN;
+    vmax[i] = NAN;
+    apd90[i] = NAN;
+    phase[i] = 0;
+}
+
+int ApdTracker::init(int64_t n_, double thr_, const double *v, double t, hipStream_t s) {
+    if(release()) return -1;
+    n = n_;
+    thr = thr_;
+    tprev = t;
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    double **bufs[] = {&prev, &vrest, &lat, &vmax, &apd90};
+    for(double **b : bufs) MONO_CHECK(hipMalloc(b, sizeof(double) * nn));
+    MONO_CHECK(hipMalloc(&phase, nn));
+    if(n > 0) {
+        k_apd_init<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, v, prev, vrest, lat, vmax, apd90, phase);
+        MONO_CHECK(hipGetLastError());
+    }
+    MONO_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int ApdTracker::release() {
+    void *bufs[] = {prev, vrest, lat, vmax, apd90, phase};
+    for(void *p : bufs)
+        if(p) MONO_CHECK(hipFree(p));
+    prev = vrest = lat = vmax = apd90 = nullptr;
+    phase = nullptr;
+    return 0;
+}
+
+int ApdTracker::sample(const double *v, const uint8_t *active, double t, hipStream_t s) {
+    if(!prev) {
+        mono_set_error("APD90 tracker not set up");
+        return -1;
+    }
+    if(n > 0) {
+        k_apd_sample<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, v, active, t, tprev, thr, prev, vrest, lat, vmax, apd90, phase);
+        MONO_CHECK(hipGetLastError());
+    }
+    tprev = t;
+    return 0;
+}
+
 }  // namespace mono

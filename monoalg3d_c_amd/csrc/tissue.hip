@@ -36,7 +36,8 @@ using namespace mono;
 
 static double alpha_of(const mono_tissue_desc &d) {
     // ALPHA (monodomain/constants.h:10-11); the 1e-8 factor is a float literal there
-    return (((d.beta * d.cm) / d.dt_pde) * 0.00000001f) * ((d.dx * d.dy) * d.dz);
+    const double dt = d.alpha_dt > 0 ? d.alpha_dt : d.dt_pde;
+    return (((d.beta * d.cm) / dt) * 0.00000001f) * ((d.dx * d.dy) * d.dz);
 }
 
 static int validate(const mono_tissue_desc *d) {
@@ -65,8 +66,8 @@ static int validate(const mono_tissue_desc *d) {
             mono_set_error("the adaptive ODE solver runs CRN 1998 (model %d given)", d->model);
             return -1;
         }
-        if(!(d->dt_ode > 0) || d->dt_ode > d->dt_pde || d->ode_abs_tol < 0 || d->ode_rel_tol < 0) {
-            mono_set_error("adaptive ODE solver: need 0 < dt_ode <= dt_pde and non-negative tolerances");
+        if(d->dt_ode < 0 || d->dt_ode > d->dt_pde || d->ode_abs_tol < 0 || d->ode_rel_tol < 0) {
+            mono_set_error("adaptive ODE solver: need 0 <= dt_ode <= dt_pde and non-negative tolerances");
             return -1;
         }
     }
@@ -172,7 +173,9 @@ int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out) {
     t->neq = model_neq(d.model);
     t->pitch = (size_t)((t->n + 63) / 64 * 64);
     t->alpha = alpha_of(d);
-    t->dt_ode = d.dt_ode > 0 ? d.dt_ode : d.dt_pde;
+    // [ode_solver] dt left at 0 (configure_ode_solver_from_options,
+    // ode_solver.c:311-336): 0.01 ms for fixed steps, min dt 1e-10 adaptive
+    t->dt_ode = d.dt_ode > 0 ? d.dt_ode : (d.ode_adaptive ? 1e-10 : 0.01);
     // sub-steps per PDE step (monodomain_solver.c:510-520); one adaptive call
     t->num_ode_steps = d.ode_adaptive ? 1 : (d.dt_pde >= t->dt_ode ? (int)(d.dt_pde / t->dt_ode) : 1);
     if(d.ode_adaptive) {
@@ -305,6 +308,7 @@ int mono_tissue_destroy(mono_tissue *t) {
     if(t->cg.release()) rc = -1;
     if(t->op.release()) rc = -1;
     if(t->obs.release()) rc = -1;
+    if(t->apd.release()) rc = -1;
     void *bufs[] = {t->sv, t->v, t->b, t->active, t->g, t->tensor, t->stim_mask, t->ionic_scale, t->aux};
     for(void *p : bufs)
         if(p && hipFree(p) != hipSuccess) rc = -1;
@@ -342,6 +346,35 @@ int mono_tissue_get_activations(mono_tissue *t, int32_t *n_act, float *act, int3
     if(act) MONO_CHECK(hipMemcpyAsync(act, t->obs.act, sizeof(float) * n * k, hipMemcpyDeviceToHost, t->stream));
     if(n_apd) MONO_CHECK(hipMemcpyAsync(n_apd, t->obs.n_apd, sizeof(int32_t) * n, hipMemcpyDeviceToHost, t->stream));
     if(apd) MONO_CHECK(hipMemcpyAsync(apd, t->obs.apd, sizeof(float) * n * k, hipMemcpyDeviceToHost, t->stream));
+    MONO_CHECK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+int mono_tissue_apd_setup(mono_tissue *t, double activation_threshold) {
+    if(!t) {
+        mono_set_error("null tissue");
+        return -1;
+    }
+    return t->apd.init(t->n, activation_threshold, t->v, t->cur_time, t->stream);
+}
+
+int mono_tissue_apd_sample(mono_tissue *t) {
+    if(!t) {
+        mono_set_error("null tissue");
+        return -1;
+    }
+    return t->apd.sample(t->v, t->active, t->cur_time, t->stream);
+}
+
+int mono_tissue_apd_get(mono_tissue *t, double *lat, double *vmax, double *apd90) {
+    if(!t || !t->apd.prev) {
+        mono_set_error("APD90 tracker not set up");
+        return -1;
+    }
+    const size_t b = sizeof(double) * (size_t)t->n;
+    if(lat) MONO_CHECK(hipMemcpyAsync(lat, t->apd.lat, b, hipMemcpyDeviceToHost, t->stream));
+    if(vmax) MONO_CHECK(hipMemcpyAsync(vmax, t->apd.vmax, b, hipMemcpyDeviceToHost, t->stream));
+    if(apd90) MONO_CHECK(hipMemcpyAsync(apd90, t->apd.apd90, b, hipMemcpyDeviceToHost, t->stream));
     MONO_CHECK(hipStreamSynchronize(t->stream));
     return 0;
 }
@@ -491,6 +524,19 @@ int mono_tissue_get_v(mono_tissue *t, double *host_v) {
 int mono_tissue_set_v(mono_tissue *t, const double *host_v) {
     if(!t || !host_v) return -1;
     MONO_CHECK(hipMemcpyAsync(t->v, host_v, sizeof(double) * t->n, hipMemcpyHostToDevice, t->stream));
+    // the first step reads V from state row 0 (the ODE solver's own initial
+    // state, as the reference's first step has no Vm mirror): a potential set
+    // before it goes there too, in the state's precision
+    if(t->st.steps == 0) {
+        if(t->d.precision == MONO_FP64) {
+            MONO_CHECK(hipMemcpyAsync(t->sv, host_v, sizeof(double) * t->n, hipMemcpyHostToDevice, t->stream));
+        } else {
+            std::vector<float> f(t->n);
+            for(int64_t i = 0; i < t->n; i++) f[i] = (float)host_v[i];
+            MONO_CHECK(hipMemcpyAsync(t->sv, f.data(), sizeof(float) * t->n, hipMemcpyHostToDevice, t->stream));
+            MONO_CHECK(hipStreamSynchronize(t->stream));
+        }
+    }
     MONO_CHECK(hipStreamSynchronize(t->stream));
     return 0;
 }
@@ -628,6 +674,82 @@ int mono_ionic_run(int model, int precision, double *states, int64_t n, const do
     MONO_CHECK(hipFree(dsv));
     MONO_CHECK(hipFree(dv));
     if(dstim) MONO_CHECK(hipFree(dstim));
+    return 0;
+}
+
+// GPU side of the ODE-solver plugin ABI: the state lives on the device
+// between calls, row 0 of it is Vm (the core mirrors cell->v into that row and
+// update_monodomain reads it, monodomain_solver.c:1023-1046,
+// update_monodomain.c:31-48).
+int mono_ode_gpu_alloc(int model, int64_t n, double **sv, size_t *pitch_bytes) {
+    if(model < 0 || model >= kNumModels || n <= 0 || !sv || !pitch_bytes) {
+        mono_set_error("mono_ode_gpu_alloc: bad arguments");
+        return -1;
+    }
+    if(mono_device_count() < 1) {
+        mono_set_error("no HIP device available");
+        return -2;
+    }
+    const int neq = model_neq(model);
+    const size_t pitch = (size_t)((n + 63) / 64 * 64);
+    double ic[64];
+    default_initial_state(model, ic);
+    double *d = nullptr, *dic = nullptr;
+    MONO_CHECK(hipMalloc(&d, sizeof(double) * pitch * neq));
+    hipError_t e = hipMalloc(&dic, sizeof(double) * neq);
+    if(e == hipSuccess) e = hipMemcpy(dic, ic, sizeof(double) * neq, hipMemcpyHostToDevice);
+    if(e == hipSuccess) e = launch_ionic_init(model, d, pitch, nullptr, n, dic, nullptr);
+    if(e == hipSuccess) e = hipDeviceSynchronize();
+    if(dic) (void)hipFree(dic);
+    if(e != hipSuccess) {
+        (void)hipFree(d);
+        mono_set_error("mono_ode_gpu_alloc: %s", hipGetErrorString(e));
+        return -1;
+    }
+    *sv = d;
+    *pitch_bytes = sizeof(double) * pitch;
+    return 0;
+}
+
+int mono_ode_gpu_solve(int model, double *sv, size_t pitch_bytes, int64_t n, const double *stim, double dt, int num_steps) {
+    if(model < 0 || model >= kNumModels || !sv || n < 0 || num_steps < 1 || pitch_bytes % sizeof(double) ||
+       pitch_bytes / sizeof(double) < (size_t)n) {
+        mono_set_error("mono_ode_gpu_solve: bad arguments");
+        return -1;
+    }
+    if(n == 0) return 0;
+    // the per-cell stimulus of the call (a host array, as the reference's
+    // solve_model_odes_gpu receives it), kept in a buffer that grows as needed
+    static double *dstim = nullptr;
+    static int64_t cap = 0;
+    if(stim) {
+        if(cap < n) {
+            if(dstim) MONO_CHECK(hipFree(dstim));
+            dstim = nullptr;
+            cap = 0;
+            MONO_CHECK(hipMalloc(&dstim, sizeof(double) * n));
+            cap = n;
+        }
+        MONO_CHECK(hipMemcpy(dstim, stim, sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    IonicLaunch<double> a{};
+    a.sv = sv;
+    a.pitch = pitch_bytes / sizeof(double);
+    a.n = n;
+    a.stim_cell = stim ? dstim : nullptr;
+    a.nx = 1;
+    a.dx = 1.0;
+    a.dt = dt;
+    a.num_steps = num_steps;
+    a.first_step = 1;    // V from row 0
+    a.write_vm_row = 1;  // V* back to row 0
+    MONO_CHECK(launch_ionic_step(model, a, nullptr));
+    MONO_CHECK(hipDeviceSynchronize());
+    return 0;
+}
+
+int mono_device_free(void *p) {
+    if(p) MONO_CHECK(hipFree(p));
     return 0;
 }
 
@@ -823,69 +945,6 @@ int mono_cg_stencil(int nx, int ny, int nz, const double *coef27, const uint8_t 
     if(error) *error = st.error;
     if(ws.release() || op.release()) return -1;
     void *bufs[] = {dcoef, db, dx, dact};
-    for(void *p : bufs)
-        if(p) MONO_CHECK(hipFree(p));
-    return 0;
-}
-
-int mono_cg_csr(int n, const int64_t *row_ptr, const int32_t *col, const double *val, const double *b, double *x, double tol,
-                int max_its, int jacobi, int *iterations, double *error) {
-    if(n <= 0 || !row_ptr || !col || !val || !b || !x) {
-        mono_set_error("mono_cg_csr: bad arguments");
-        return -1;
-    }
-    if(mono_device_count() < 1) {
-        mono_set_error("no HIP device available");
-        return -2;
-    }
-    const int64_t nnz = row_ptr[n];
-    for(int64_t k = 0; k < nnz; k++)
-        if(col[k] < 0 || col[k] >= n) {
-            mono_set_error("mono_cg_csr: column index %d out of range", col[k]);
-            return -1;
-        }
-    std::vector<double> diag(n, 0.0);
-    for(int i = 0; i < n; i++)
-        for(int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++)
-            if(col[k] == i) diag[i] = val[k];
-    int64_t *drp = nullptr;
-    int32_t *dcol = nullptr;
-    double *dval = nullptr, *db = nullptr, *dx = nullptr, *ddiag = nullptr;
-    MONO_CHECK(hipMalloc(&drp, sizeof(int64_t) * (n + 1)));
-    MONO_CHECK(hipMalloc(&dcol, sizeof(int32_t) * (nnz > 0 ? nnz : 1)));
-    MONO_CHECK(hipMalloc(&dval, sizeof(double) * (nnz > 0 ? nnz : 1)));
-    MONO_CHECK(hipMalloc(&db, sizeof(double) * n));
-    MONO_CHECK(hipMalloc(&dx, sizeof(double) * n));
-    MONO_CHECK(hipMalloc(&ddiag, sizeof(double) * n));
-    MONO_CHECK(hipMemcpy(drp, row_ptr, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
-    if(nnz > 0) {
-        MONO_CHECK(hipMemcpy(dcol, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
-        MONO_CHECK(hipMemcpy(dval, val, sizeof(double) * nnz, hipMemcpyHostToDevice));
-    }
-    MONO_CHECK(hipMemcpy(db, b, sizeof(double) * n, hipMemcpyHostToDevice));
-    MONO_CHECK(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
-    MONO_CHECK(hipMemcpy(ddiag, diag.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    CsrMatrix A;
-    A.n = n;
-    A.nnz = nnz;
-    A.row_ptr = drp;
-    A.col = dcol;
-    A.val = dval;
-    A.diag = ddiag;
-    CgWorkspace ws;
-    if(ws.init(n, n)) return -1;
-    CgConfig cfg;
-    cfg.tol = tol;
-    cfg.max_its = max_its;
-    cfg.jacobi = jacobi;
-    CgStats st{};
-    if(ws.solve_csr(A, db, dx, cfg, &st, nullptr)) return -1;
-    MONO_CHECK(hipDeviceSynchronize());
-    MONO_CHECK(hipMemcpy(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost));
-    if(iterations) *iterations = st.iterations;
-    if(error) *error = st.error;
-    if(ws.release()) return -1;
-    void *bufs[] = {drp, dcol, dval, db, dx, ddiag};
     for(void *p : bufs)
         if(p) MONO_CHECK(hipFree(p));
     return 0;

@@ -27,6 +27,7 @@ struct mono_tissue {
     mono::CgWorkspace cg;
     mono::CgConfig cgcfg;
     mono::Observer obs;
+    mono::ApdTracker apd;
     hipStream_t stream = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     int timing = 0;

@@ -162,6 +162,46 @@ def read_fibres(path: str) -> np.ndarray:
     return np.array(rows)
 
 
+def write_custom_mesh(path: str, shape, h: float, active=None) -> int:
+    """A custom mesh file (initialize_grid_with_custom_mesh, domain_helpers.c:95-278)
+    listing the active cells of a box in lexicographic order, one
+    "cx,cy,cz,h/2,h/2,h/2" line each; returns the number of lines."""
+    nx, ny, nz = shape
+    xs = [f"{(x + 0.5) * h:.1f}" for x in range(nx)]
+    half = f"{h / 2}"
+    count = 0
+    with open(path, "w") as fh:
+        for z in range(nz):
+            cz = f"{(z + 0.5) * h:.1f}"
+            for y in range(ny):
+                cy = f"{(y + 0.5) * h:.1f}"
+                tail = f",{cy},{cz},{half},{half},{half}\n"
+                if active is None:
+                    fh.write("".join(x + tail for x in xs))
+                    count += nx
+                else:
+                    row = active[(z * ny + y) * nx:(z * ny + y + 1) * nx]
+                    fh.write("".join(xs[x] + tail for x in np.nonzero(row)[0]))
+                    count += int(row.sum())
+    return count
+
+
+def write_layered_fibres(path: str, shape, theta0_deg: float, theta1_deg: float, active=None) -> None:
+    """The fibre file matching write_custom_mesh's lines: f = (cos t, sin t, 0),
+    s = (-sin t, cos t, 0), n = e_z with t rising linearly over depth from
+    theta0 to theta1 (endpoints included), one "f s n" line per listed cell."""
+    nx, ny, nz = shape
+    with open(path, "w") as fh:
+        for z in range(nz):
+            th = math.radians(theta0_deg + (theta1_deg - theta0_deg) * (z / (nz - 1) if nz > 1 else 0.0))
+            c, s_ = math.cos(th), math.sin(th)
+            line = "%.17g %.17g 0 %.17g %.17g 0 0 0 1\n" % (c, s_, -s_, c)
+            if active is None:
+                fh.write(line * (nx * ny))
+            else:
+                fh.write(line * int(active[z * nx * ny:(z + 1) * nx * ny].sum()))
+
+
 def calc_tensor(f, s, n, sl, st, sn) -> np.ndarray:
     """D = sl f f^T + st s s^T + sn n n^T, summed as (ff + ss) + nn
     (assembly_common.c:1102-1119); returns (xx, yy, zz, xy, xz, yz)."""
@@ -277,10 +317,26 @@ def load_ini(path: str) -> Simulation:
     if lib not in MODEL_LIBS:
         raise ConfigError(f"ionic model {lib} is not available (models: {', '.join(MODEL_LIBS)})")
     desc["model"] = MODEL_LIBS[lib]
-    desc["dt"] = dt_pde
-    desc["dt_ode"] = num(ode.get("dt", dt_pde))
+    # configure_ode_solver_from_options (ode_solver.c:306-336): [ode_solver] dt
+    # defaults to 0 (config_parser.c:297); a fixed-step run then steps 0.01 ms,
+    # an adaptive one (or auto_dt) starts from min dt 1e-10 with max dt = dt_pde;
+    # tolerances are `abstol` / `reltol` (config_parser.c:1648-1651, defaults 1e-5)
+    dt_ode = num(ode.get("dt", 0.0))
     if flag(ode.get("adaptive", "no")):
-        desc["adaptive"] = (num(ode.get("abs_tol", 1e-5)), num(ode.get("rel_tol", 1e-5)))
+        if flag(ode.get("auto_dt", "no")) or dt_ode == 0.0:
+            dt_ode = 1e-10
+        tol = lambda key, alt: num(ode.get(key, ode.get(alt, 1e-5)))
+        desc["adaptive"] = (tol("abstol", "abs_tol"), tol("reltol", "rel_tol"))
+    else:
+        if dt_ode == 0.0:
+            dt_ode = 0.01
+        if dt_pde < dt_ode:
+            # the loop steps dt_ode, ALPHA keeps the configured dt_pde
+            # (monodomain_solver.c:512-519,1326)
+            desc["alpha_dt"] = dt_pde
+            dt_pde = dt_ode
+    desc["dt"] = dt_pde
+    desc["dt_ode"] = dt_ode
     desc["beta"] = num(main.get("beta", 0.14))
     desc["cm"] = num(main.get("cm", 1.0))
 

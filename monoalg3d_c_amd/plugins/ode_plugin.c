@@ -15,7 +15,10 @@
  * and sv is the reference's array-of-structures layout sv[cell*neq + s].
  * With the CPU core the states live on the host; each call uploads them, runs
  * the HIP kernel and downloads them (the device-resident whole step is
- * mono_tissue_step).
+ * mono_tissue_step).  The GPU pair (set_model_initial_conditions_gpu /
+ * solve_model_odes_gpu, the core's use_gpu path) keeps the states on the
+ * device between calls as the reference's GPU models do: SoA rows of `pitch`
+ * bytes, row 0 = Vm.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -89,4 +92,35 @@ SOLVE_MODEL_ODES(solve_model_odes_cpu) {
     if(mono_ionic_run(MONO_PLUGIN_MODEL, MONO_FP64, ode_solver->sv, (int64_t)ode_solver->num_cells_to_solve, stim_currents,
                       ode_solver->min_dt, (int)ode_solver->num_steps))
         fail("mono_ionic_run");
+}
+
+/* SET_ODE_INITIAL_CONDITIONS_GPU (ode_solver.h:46; e.g. ten_tusscher_2006_RS_GPU.cu:8-40):
+ * device sv, every cell at the default initial state; returns the row pitch
+ * in bytes (the core stores it in solver->pitch and frees sv at the end) */
+SET_ODE_INITIAL_CONDITIONS_GPU(set_model_initial_conditions_gpu) {
+    (void)ode_extra_config;
+    if(solver->adaptive) {
+        fprintf(stderr, "libmono_hip: the GPU ODE path runs fixed steps (adaptive=true runs through the CPU symbols)\n");
+        exit(EXIT_FAILURE);
+    }
+    double *sv = NULL;
+    size_t pitch = 0;
+    if(mono_ode_gpu_alloc(MONO_PLUGIN_MODEL, (int64_t)solver->original_num_cells, &sv, &pitch)) fail("mono_ode_gpu_alloc");
+    solver->sv = sv;
+    return pitch;
+}
+
+/* SOLVE_MODEL_ODES on the device (solve_model_odes_gpu, e.g.
+ * ten_tusscher_2006_RS_GPU.cu:42-156): num_steps steps of min_dt for every
+ * cell, stim_currents a host array in cell order */
+SOLVE_MODEL_ODES(solve_model_odes_gpu) {
+    (void)ode_extra_config;
+    (void)current_t;
+    if(ode_solver->cells_to_solve) {
+        fprintf(stderr, "libmono_hip: adaptive cell lists are not supported\n");
+        exit(EXIT_FAILURE);
+    }
+    if(mono_ode_gpu_solve(MONO_PLUGIN_MODEL, ode_solver->sv, ode_solver->pitch, (int64_t)ode_solver->num_cells_to_solve,
+                          stim_currents, ode_solver->min_dt, (int)ode_solver->num_steps))
+        fail("mono_ode_gpu_solve");
 }

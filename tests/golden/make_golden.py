@@ -497,6 +497,58 @@ def solvers(tmp):
                                   "test_solvers.c with a 1e-3 absolute tolerance), converted to COO arrays")
 
 
+SWEEP_LENGTHS = ["2000", "3500", "5000"]
+SWEEP_XLIMITS = ["500", "1000"]
+
+
+def sweep_batch_files(out_dir):
+    """The base run (a TT2006 cable of 5000 um, 40 ms, Vm every 50 steps) and a
+    [batch] file sweeping domain|cable_length and stim_plain|x_limit, as the
+    reference's batch mode reads them (main_batch.c:17-212).  Returns (batch
+    file, base .ini)."""
+    ini, _, _ = case_files(os.path.join(out_dir, "base"), (50, 1, 1), "tt2006", 40.0, 50, cable=True)
+    batch = os.path.join(out_dir, "sweep.ini")
+    with open(batch, "w") as fh:
+        fh.write(f"[batch]\ninitial_config={ini}\noutput_folder={os.path.join(out_dir, 'out')}\n"
+                 "num_simulations_per_parameter_change=1\n[modify]\n"
+                 f"domain|cable_length=list|{'|'.join(SWEEP_LENGTHS)}\nstim_plain|x_limit=list|{'|'.join(SWEEP_XLIMITS)}\n")
+    return batch, ini
+
+
+def sweep(tmp):
+    """Every run of the sweep above on the reference binary, one MonoAlg3D run
+    per expanded configuration (what each MPI rank of main_batch runs): runs
+    numbered from 1 over the product of the lists, the last varying fastest
+    (batch_utils.c:215-296)."""
+    batch, ini = sweep_batch_files(os.path.join(tmp, "sweep"))
+    base = open(ini).read()
+    fx, k = {}, 1
+    runs = []
+    for length in SWEEP_LENGTHS:
+        for xl in SWEEP_XLIMITS:
+            d = os.path.join(tmp, "sweep", f"ref_run_{k}")
+            text = base.replace("cable_length=5000.0", f"cable_length={length}").replace("x_limit = 500.0", f"x_limit = {xl}")
+            text = text.replace(f"output_dir={os.path.join(tmp, 'sweep', 'base')}", f"output_dir={d}")
+            assert f"cable_length={length}" in text and f"x_limit = {xl}" in text and f"output_dir={d}" in text
+            os.makedirs(d, exist_ok=True)
+            path = os.path.join(d, "run.ini")
+            with open(path, "w") as fh:
+                fh.write(text)
+            run([os.path.join(REF, "bin", "MonoAlg3D"), "-c", path])
+            n = int(round(float(length) / H))
+            snaps = read_snapshots(d, (n, 1, 1), 50, int(round(40.0 / DT)) + 1)
+            fx[f"run{k}_vm"] = snaps
+            fx[f"run{k}_lat"] = lat_from(snaps, 50)
+            runs.append((k, float(length), float(xl)))
+            k += 1
+    fx["runs"] = np.array(runs)
+    np.savez_compressed(os.path.join(HERE, "sweep_cable_tt2006.npz"), **fx)
+    PROV["sweep_cable_tt2006.npz"] = ("reference MonoAlg3D (oracle/_ref) run once per run of a [batch] sweep of a "
+                                      "TT2006 cable (5000 um base, isotropic 0.13 S/m, CG tol 1e-16, 40 ms, Vm every "
+                                      "50 steps): domain|cable_length in {2000,3500,5000} x stim_plain|x_limit in "
+                                      "{500,1000}, runs numbered as main_batch does (last list fastest)")
+
+
 def main():
     groups = sys.argv[1:] or ["ionic", "assembly", "tissue", "solvers"]
     if not os.path.exists(os.path.join(REF, "bin", "ref_harness")):
@@ -527,6 +579,8 @@ def main():
             tissue(tmp, "sheet_tt2006_48x32", (48, 32, 1), "tt2006", 40.0, 50, keep_times=(20, 40))
         if "checkpoint" in groups:
             checkpoint(tmp)
+        if "sweep" in groups:
+            sweep(tmp)
         if "config1" in groups:
             tissue(tmp, "config1_tt2006_64x64x16", (64, 64, 16), "tt2006", 50.0, 10, keep_times=(20,))
     finally:

@@ -164,10 +164,62 @@ REF_EXE = os.path.join(ROOT, "oracle", "_ref", "bin", "MonoAlg3D")
 @pytest.mark.skipif("not (os.path.exists(PLUGINS) and os.path.exists(REF_EXE) and _no_device())")
 def test_plugins_bind_to_reference_core(tmp_path):
     """The reference's own MonoAlg3D loads the plugins through its dlopen ABI:
-    the model queries and initial conditions run on the host, and the first ODE
-    solve reaches libmonoalg_hip.so, which refuses without a GPU."""
+    the model queries and initial conditions run on the host, and the first
+    device call (the linear solver's INIT_LINEAR_SYSTEM, which uploads the
+    matrix) reaches libmonoalg_hip.so, which refuses without a GPU."""
     ini = _plugin_ini(tmp_path, (8, 8, 2), 1.0, 5)
     r = subprocess.run([REF_EXE, "-c", ini], cwd=os.path.dirname(os.path.dirname(REF_EXE)), capture_output=True, text=True,
                        timeout=120)
     assert r.returncode != 0
-    assert "libmono_hip: mono_ionic_run failed: no HIP device available" in r.stderr, r.stderr[-2000:]
+    assert "libmono_hip: mono_csr_create failed: no HIP device available" in r.stderr, r.stderr[-2000:]
+
+
+def test_malformed_csr_is_refused_before_any_device_work():
+    """A CSR whose row_ptr does not start at 0, decreases, or whose columns
+    leave the matrix is refused by the C side (-1 and a message) before the
+    host or the device index col/val through it; the Python wrappers check the
+    array lengths as well."""
+    import monoalg3d_c_amd as m
+    from monoalg3d_c_amd import _lib
+    L = _lib.lib()
+    b = np.ones(3)
+    x = np.zeros(3)
+    val = np.ones(5)
+    its, err = C.c_int(), C.c_double()
+
+    def call(rp, col):
+        rp = np.asarray(rp, np.int64)
+        col = np.asarray(col, np.int32)
+        return L.mono_cg_csr(3, rp.ctypes.data, col.ctypes.data, val.ctypes.data, b.ctypes.data, x.ctypes.data, 1e-16,
+                             10, 0, C.byref(its), C.byref(err))
+    assert call([1, 2, 3, 5], [0, 1, 2, 0, 1]) == -1 and b"row_ptr[0]" in L.mono_last_error()
+    assert call([0, 3, 2, 5], [0, 1, 2, 0, 1]) == -1 and b"decreases" in L.mono_last_error()
+    assert call([0, 2, 3, 5], [0, 1, 2, 0, 7]) == -1 and b"column index 7" in L.mono_last_error()
+    h = C.c_void_p()
+    rp = np.array([0, 2, 1, 5], np.int64)
+    col = np.zeros(5, np.int32)
+    assert L.mono_csr_create(3, rp.ctypes.data, col.ctypes.data, val.ctypes.data, C.byref(h)) == -1
+    with pytest.raises(ValueError, match="row_ptr"):
+        m.cg_csr([0, 2, 1, 5], [0, 1, 2, 0, 1], val, b, x)
+    with pytest.raises(ValueError, match="col and val"):
+        m.cg_csr([0, 2, 3, 5], [0, 1, 2, 0], val, b, x)
+    with pytest.raises(ValueError, match="b has 4 entries"):
+        m.cg_csr([0, 2, 3, 5], [0, 1, 2, 0, 1], val, np.ones(4), x)
+
+
+def test_wrappers_refuse_mis_sized_arrays():
+    """States narrower than the model, or a stimulus of the wrong length, are
+    refused before the library writes states[i * neq + s] (host heap)."""
+    import monoalg3d_c_amd as m
+    with pytest.raises(ValueError, match=r"shape \(n, 19\)"):
+        m.ionic_run(m.MODEL_TT2006, np.zeros((4, 18)), None, 0.02)
+    with pytest.raises(ValueError, match="stim has 3 entries"):
+        m.ionic_run(m.MODEL_TT2006, np.zeros((4, 19)), np.zeros(3), 0.02)
+    with pytest.raises(ValueError, match="aux"):
+        m.ionic_run_adaptive(m.MODEL_CRN1998, np.zeros((2, 21)), np.zeros((3, 3)), None, 0.0, 0.01, 0.02)
+    with pytest.raises(ValueError, match="coef27"):
+        m.cg_stencil((2, 2, 1), np.zeros((4, 26)), np.ones(4), np.zeros(4))
+    with pytest.raises(ValueError, match="entries"):
+        m.cg_stencil((2, 2, 1), np.zeros((4, 27)), np.ones(3), np.zeros(4))
+    with pytest.raises(ValueError, match="alpha and vm"):
+        m.update_rhs(np.ones(3), np.ones(4))

@@ -160,3 +160,105 @@ def test_sweep_file_runs_match_standalone_runs(tmp_path):
             want.append(t.v)
         t.close()
         np.testing.assert_allclose(snaps, np.array(want), rtol=0, atol=VM_TOL, err_msg=f"run {num_run}")
+
+
+def test_sweep_matches_reference_runs(tmp_path):
+    """The reference's own batch sweep: a [batch] file over cable length x
+    stimulus x_limit (tests/golden/make_golden.py sweep_batch_files), run by
+    run_sweep as one atlas, against MonoAlg3D run once per expanded
+    configuration (what each MPI rank of main_batch runs): every run's
+    V_it files within VM_TOL of that run's reference Vm, activation times
+    within 1e-3 ms, run numbers and output directories as main_batch names them."""
+    from make_golden import SWEEP_LENGTHS, SWEEP_XLIMITS, read_snapshots, sweep_batch_files
+    from monoalg3d_c_amd.sweep import plan, run_sweep
+    announce("sweep file vs the reference's per-run outputs: 3 cable lengths x 2 stimulus x limits")
+    g = np.load(os.path.join(GOLD, "sweep_cable_tt2006.npz"))
+    batch, ini = sweep_batch_files(str(tmp_path))
+    groups = plan(batch)
+    assert len(groups) == 1 and len(groups[0]) == 6  # one atlas for the whole sweep
+    done = run_sweep(batch)
+    assert [d[0] for d in done] == [1, 2, 3, 4, 5, 6]
+    k = 0
+    for length in SWEEP_LENGTHS:
+        for xl in SWEEP_XLIMITS:
+            num, out = done[k]
+            k += 1
+            assert out.endswith(f"_run_{num}_cable_length_{length}_x_limit_{xl}"), out
+            n = int(round(float(length) / 100.0))
+            snaps = read_snapshots(out, (n, 1, 1), 50, int(round(40.0 / 0.02)) + 1)
+            ref = g[f"run{num}_vm"]
+            assert snaps.shape == ref.shape, (num, snaps.shape, ref.shape)
+            dv = np.abs(snaps - ref).max()
+            assert dv < VM_TOL, (num, dv)
+            lat = lat_from(snaps, 50)
+            assert np.array_equal(np.isfinite(lat), np.isfinite(g[f"run{num}_lat"])), num
+            assert np.abs(lat - g[f"run{num}_lat"]).max() <= 1e-3, num
+
+
+def test_batch_relative_tolerance_bounds_every_instance():
+    """cg_rel_tol on a batch: the atlas stops on the absolute bound
+    rel^2 * min_i n_i * (alpha * 1 mV)^2, so a 100-cell cable packed beside a
+    192x192 sheet is solved at least as tightly as alone with the relative
+    test: after every step of 10 ms the atlas residual r.r is below
+    rel^2 ||b_cable||^2 (b = alpha V*), and both instances match their
+    stand-alone runs with the relative test within VM_TOL."""
+    import monoalg3d_c_amd as m
+    from monoalg3d_c_amd.batch import Batch, Instance
+    announce("batch with cg_rel_tol: small cable beside a large sheet, 10 ms")
+    insts = [Instance(100, 1), Instance(192, 192)]
+    rel = 1e-10
+    a = m.alpha()
+    b = Batch(insts, precision=m.FP64, cg_rel_tol=rel, tensor=m.iso_tensor(0.00013))
+    for _ in range(500):
+        b.step(1)
+        cable = b.v()[0]
+        assert b.stats["last_cg_error"] <= rel * rel * float(((a * cable) ** 2).sum())
+    got = b.v()
+    b.close()
+    for i, inst in enumerate(insts):
+        t = m.Tissue((inst.nx, inst.ny, 1), precision=m.FP64, cg_rel_tol=rel, cg_tol=0.0, tensor=m.iso_tensor(0.00013))
+        t.step(500)
+        dv = np.abs(got[i] - t.v).max()
+        t.close()
+        assert dv < VM_TOL, (i, dv)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_config4_256_instances_500ms(precision):
+    """Config 4 at its stated size: the seed-4 sweep of 256 instances (cables
+    of 100-2000 cells, square sheets of 32-256 cells a side, TT2006 with GKr,
+    GCaL, GKs x U[0.5, 1.5]), 1 Hz pacing, 500 ms, CG relative residual 1e-10
+    per instance.  No NaN; every instance activates; per-instance activation
+    times of FP32+refine within 0.5 ms of the FP64 batch (the FP64 batch is the
+    reference for the scaled instances; scale-1 instances are pinned to the
+    reference's runs above)."""
+    import monoalg3d_c_amd as m
+    from monoalg3d_c_amd.batch import Batch, sweep_instances
+    insts = sweep_instances(256, 4)
+    announce(f"config 4: 256 instances, 500 ms, {precision} (+ fp64 for the comparison)")
+    out = {}
+    for key, prec in (("fp64", m.FP64), ("fp32", m.FP32_REFINE)):
+        if key != "fp64" and precision == "fp64":
+            continue
+        b = Batch(insts, precision=prec, cg_rel_tol=1e-10, cg_max_its=500, tensor=m.iso_tensor(0.00013),
+                  stim=(0.0, 2.0, -50.0, 500.0, 1000.0))
+        b.observe_setup(activation_threshold=-30.0, apd_threshold=-83.0, time_threshold=10.0, max_events=2)
+        for k in range(25001):
+            if k % 5 == 0:
+                b.observe()
+            if b.step(1) == 1:
+                break
+        st = b.stats
+        acts = b.activations()
+        b.close()
+        assert st["nan_detected"] == 0, key
+        out[key] = acts
+    for i, a in enumerate(out["fp64"]):
+        assert (a["n_act"] >= 1).all(), f"instance {i} did not activate everywhere"
+    if precision == "fp32":
+        worst = 0.0
+        for a64, a32 in zip(out["fp64"], out["fp32"]):
+            np.testing.assert_array_equal(a64["n_act"], a32["n_act"])
+            worst = max(worst, float(np.abs(a64["act"][:, 0] - a32["act"][:, 0]).max()))
+        print(f"[config4] max activation-time difference fp32+refine vs fp64: {worst:.4f} ms", flush=True)
+        assert worst <= 0.5, worst

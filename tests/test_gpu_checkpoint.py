@@ -200,3 +200,45 @@ def test_ini_restart_from_reference_checkpoint(fx, tmp_path):
     assert np.abs(got - fx["restart_vm"]).max() < VM_TOL
     assert not os.path.exists(os.path.join(out, "V_it_300.bin"))
     assert os.path.exists(os.path.join(out, "ck2", "hip_solver_checkpoint.dat"))
+
+
+_CROSS_PROCESS = r"""
+import sys
+import numpy as np
+sys.path.insert(0, {root!r})
+import monoalg3d_c_amd as m
+shape = (96, 80, 12)
+t = m.Tissue(shape, model=m.MODEL_ORD2011, precision=getattr(m, {prec!r}), aniso=True,
+             tensor=m.layered_fibre_tensors(12, -60.0, 60.0, 0.00024, 0.000045), tensor_source=m.TENSOR_PER_LAYER,
+             cg_rel_tol=1e-10, cg_tol=0.0, cg_max_its=500)
+mode = {mode!r}
+if mode == "restore":
+    t.restore_state({ck!r})
+t.step({steps})
+if mode == "save":
+    t.save_state({ck!r})
+np.save({out!r}, t.v)
+t.close()
+"""
+
+
+@pytest.mark.parametrize("prec", ["FP64", "FP32_REFINE"])
+def test_restart_in_another_process_is_bitwise(prec, tmp_path):
+    """Three processes: one runs 400 steps uninterrupted; one runs 200 and
+    saves; a third restores and runs 200 more.  The plane-loop kernels' work
+    split and partial-sum order are a function of the box and the CU count
+    only (not of how many blocks a process measured resident), so the
+    restarted run repeats the uninterrupted one bit for bit across processes.
+    The box (96x80x12, ORd, rotating fibres) uses the 3-D tiled kernels."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    announce(f"cross-process restart, 96x80x12 ORd {prec}")
+    ck = str(tmp_path / "ck")
+    os.makedirs(ck)
+    runs = {"full": ("none", 400), "first": ("save", 200), "second": ("restore", 200)}
+    for name, (mode, steps) in runs.items():
+        code = _CROSS_PROCESS.format(root=root, prec=prec, mode=mode, ck=ck, steps=steps, out=str(tmp_path / f"{name}.npy"))
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, (name, r.stderr[-2000:])
+    full, second = np.load(str(tmp_path / "full.npy")), np.load(str(tmp_path / "second.npy"))
+    np.testing.assert_array_equal(second, full)

@@ -61,9 +61,13 @@ def _check(g, snaps, st, vm_tol, lat_tol, vm_times):
     assert np.array_equal(np.isfinite(lat[act]), np.isfinite(ref[act])), "different cells activated"
     dlat = np.abs(lat[both] - ref[both]).max() if both.any() else 0.0
     assert st["nan_detected"] == 0
+    dmaps = []
     for tms in vm_times:
         k = int(round(tms / (int(g["every"]) * DT)))
-        dmap = np.abs(snaps[k][act] - g[f"vm_t{int(tms)}"][act]).max()
+        dmaps.append(float(np.abs(snaps[k][act] - g[f"vm_t{int(tms)}"][act]).max()))
+    print(f"[parity] probe max |dV| = {dv:.3e} mV, map max |dV| = {max(dmaps, default=0.0):.3e} mV "
+          f"(relative to the 100 mV Vm scale: {max([dv] + dmaps) / 100.0:.2e}), max |dLAT| = {dlat:.3e} ms", flush=True)
+    for tms, dmap in zip(vm_times, dmaps):
         assert dmap < vm_tol, (tms, dmap)
     assert dv < vm_tol, dv
     assert dlat <= lat_tol, dlat

@@ -119,3 +119,79 @@ def test_rejects_unsupported(tmp_path):
     bad.write_text(text.replace("libten_tusscher_2006", "libluo_rudy_1991"))
     with pytest.raises(ConfigError, match="not available"):
         load_ini(str(bad))
+
+
+def _with_ode_section(tmp_path, body):
+    """The cuboid test .ini with its [ode_solver] section replaced by `body`."""
+    import re
+    ini, act, fib = case_files(str(tmp_path), (8, 8, 2), "tt2006", 1.0, 5)
+    text = open(ini).read()
+    text = re.sub(r"\[ode_solver\][^\[]*", "[ode_solver]\n" + body + "\n", text)
+    p = tmp_path / "ode.ini"
+    p.write_text(text)
+    return str(p)
+
+
+def test_ode_solver_keys_and_defaults(tmp_path):
+    """[ode_solver] as the reference reads it (config_parser.c:297-311,
+    1645-1668; configure_ode_solver_from_options, ode_solver.c:306-336;
+    monodomain_solver.c:510-519):
+      - no dt: fixed-step runs step 0.01 ms (0.02 ms PDE step -> 2 ODE steps);
+      - adaptive without dt (or auto_dt): min dt 1e-10, max dt = dt_pde;
+      - adaptive tolerances are `abstol` / `reltol` (default 1e-5 each);
+      - dt above dt_pde: the loop steps dt, ALPHA keeps the configured dt_pde."""
+    from monoalg3d_c_amd.inputs import load_ini
+    lib = "library_file=shared_libs/libten_tusscher_2006.so"
+    d = load_ini(_with_ode_section(tmp_path, lib)).desc
+    assert d["dt_ode"] == 0.01 and d["dt"] == 0.02 and "adaptive" not in d and "alpha_dt" not in d
+    crn = "library_file=shared_libs/libcourtemanche_ramirez_nattel_1998.so"
+    d = load_ini(_with_ode_section(tmp_path, crn + "\nadaptive=true")).desc
+    assert d["dt_ode"] == 1e-10 and d["adaptive"] == (1e-5, 1e-5)
+    d = load_ini(_with_ode_section(tmp_path, crn + "\nadaptive=true\ndt=0.001\nabstol=1e-7\nreltol=1e-6")).desc
+    assert d["dt_ode"] == 0.001 and d["adaptive"] == (1e-7, 1e-6)
+    d = load_ini(_with_ode_section(tmp_path, crn + "\nadaptive=true\ndt=0.001\nauto_dt=true")).desc
+    assert d["dt_ode"] == 1e-10
+    sim = load_ini(_with_ode_section(tmp_path, lib + "\ndt=0.05"))
+    assert sim.desc["dt"] == 0.05 and sim.desc["dt_ode"] == 0.05 and sim.desc["alpha_dt"] == 0.02
+    assert sim.dt_pde == 0.05
+
+
+def test_splitmix64_known_answer():
+    """SplitMix64 (Steele, Lea & Flood 2014): the first outputs from state 0 are
+    0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F; the uniform draw
+    is the top 53 bits."""
+    import monoalg3d_c_amd as m
+    want = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    got = m.splitmix64_uniform(0, np.arange(3))
+    np.testing.assert_array_equal(got, [(w >> 11) * 2.0 ** -53 for w in want])
+    # stream position, not call order, decides a cell's draw
+    np.testing.assert_array_equal(m.splitmix64_uniform(7, np.arange(10, 20)), m.splitmix64_uniform(7, np.arange(30))[10:20])
+    mask = m.fibrosis_mask((40, 30, 20), 0.1, 3, chunk=1000)
+    np.testing.assert_array_equal(mask, (m.splitmix64_uniform(3, np.arange(24000)) >= 0.1).astype(np.uint8))
+    assert abs((1 - mask.mean()) - 0.1) < 0.01
+
+
+def test_mesh_and_fibre_writers_read_back(tmp_path):
+    """The reference-format inputs written for a box with holes read back to the
+    same active cells and, per listed cell, the layered fibre of its depth."""
+    import monoalg3d_c_amd as m
+    from monoalg3d_c_amd.inputs import read_custom_mesh, read_fibres, write_custom_mesh, write_layered_fibres
+    shape = (9, 7, 5)
+    act = m.fibrosis_mask(shape, 0.2, 11)
+    mesh, fib = str(tmp_path / "mesh.alg"), str(tmp_path / "fib.txt")
+    count = write_custom_mesh(mesh, shape, 100.0, act)
+    write_layered_fibres(fib, shape, -60.0, 60.0, act)
+    assert count == act.sum()
+    sh, active, line = read_custom_mesh(mesh, count, 100.0)
+    f = read_fibres(fib)
+    assert len(f) == count
+    # the mesh's box is the bounding box of the listed cells
+    nx, ny, nz = sh
+    lex_back = np.nonzero(active)[0]
+    xs, ys, zs = lex_back % nx, (lex_back // nx) % ny, lex_back // (nx * ny)
+    want = np.nonzero(act)[0]
+    np.testing.assert_array_equal(np.sort(xs + shape[0] * (ys + shape[1] * zs)), want)
+    for k, c in enumerate(want):
+        z = c // (shape[0] * shape[1])
+        th = np.deg2rad(-60.0 + 120.0 * z / (shape[2] - 1))
+        np.testing.assert_allclose(f[k][:3], [np.cos(th), np.sin(th), 0.0], atol=1e-15)
