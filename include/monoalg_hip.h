@@ -116,6 +116,14 @@ typedef struct mono_tissue_desc {
      * (monodomain_solver.c:512-519) while ALPHA keeps the configured dt_pde
      * (the_monodomain_solver->dt, monodomain_solver.c:1326); 0 = dt_pde. */
     double alpha_dt;
+    /* optional per-instance model scalings (batches of many instances; no
+     * reference counterpart): the instance of each cell (n uint16, host) and a
+     * table of 4 floats per instance (n_instances x 4, host; the factors of
+     * ionic_scale), staged in LDS by the ionic kernel.  Replaces ionic_scale;
+     * at most 4096 instances. */
+    const uint16_t *instance;
+    const float *instance_scale;
+    int n_instances;
 } mono_tissue_desc;
 
 typedef struct mono_stats {

@@ -99,14 +99,20 @@ def fibrosis_mask(shape, fraction: float, seed: int, chunk: int = 1 << 22) -> np
     return out
 
 
+MAX_INSTANCES = 4096  # rows of a per-instance scaling table (kMaxInstances, ionic.h)
+
+
 def make_desc(shape, h=(100.0, 100.0, 100.0), model=MODEL_TT2006, precision=FP64, dt=0.02, dt_ode=None, beta=0.14,
               cm=1.0, aniso=False, tensor=None, tensor_source=TENSOR_UNIFORM, g_cell=None, active=None,
               cg_tol=1e-16, cg_rel_tol=0.0, cg_max_its=200, jacobi=False, inner_rel=1e-4,
-              stim=(0.0, 2.0, -50.0, 500.0, 0.0), stim_mask=None, ionic_scale=None, adaptive=None, alpha_dt=0.0):
+              stim=(0.0, 2.0, -50.0, 500.0, 0.0), stim_mask=None, ionic_scale=None, adaptive=None, alpha_dt=0.0,
+              instance=None, instance_scale=None):
     """Build a TissueDesc; returns (desc, keepalive) where keepalive holds the arrays.
 
     stim_mask (n,) 0/1 replaces the x < stim_x_limit rule; ionic_scale (n, 4)
-    float32 scales TT2006 GKr/GCaL/GKs or MS2003 tau_close per cell.
+    float32 scales TT2006 GKr/GCaL/GKs or MS2003 tau_close per cell; instead,
+    instance (n,) uint16 names each cell's row of instance_scale (k, 4),
+    k <= 4096 (the form a batch uses: the table sits in LDS).
     adaptive: None, or (abs_tol, rel_tol) for the adaptive Euler ODE solver
     (min dt = dt_ode, max dt = dt; CRN 1998).  alpha_dt > 0: the dt of ALPHA
     when it differs from the loop's dt (a dt_ode above dt_pde)."""
@@ -144,6 +150,22 @@ def make_desc(shape, h=(100.0, 100.0, 100.0), model=MODEL_TT2006, precision=FP64
             raise ValueError(f"ionic_scale has {ionic_scale.size} entries for 4 x {n} cells")
         keep.append(ionic_scale)
         d.ionic_scale = ionic_scale.ctypes.data
+    if (instance is None) != (instance_scale is None):
+        raise ValueError("instance and instance_scale go together")
+    if instance is not None:
+        if ionic_scale is not None:
+            raise ValueError("give ionic_scale or instance/instance_scale, not both")
+        instance = np.ascontiguousarray(instance, np.uint16).reshape(-1)
+        instance_scale = np.ascontiguousarray(instance_scale, np.float32).reshape(-1, 4)
+        k = instance_scale.shape[0]
+        if instance.size != n:
+            raise ValueError(f"instance has {instance.size} entries for {n} cells")
+        if not 1 <= k <= MAX_INSTANCES:
+            raise ValueError(f"instance_scale has {k} rows; 1 to {MAX_INSTANCES} allowed")
+        if int(instance.max()) >= k:
+            raise ValueError(f"instance names row {int(instance.max())} of a {k}-row table")
+        keep += [instance, instance_scale]
+        d.instance, d.instance_scale, d.n_instances = instance.ctypes.data, instance_scale.ctypes.data, k
     if adaptive is not None:
         d.ode_adaptive = 1
         d.ode_abs_tol, d.ode_rel_tol = adaptive

@@ -51,6 +51,7 @@ class TissueDesc(C.Structure):
         ("stim_mask", C.c_void_p), ("ionic_scale", C.c_void_p),
         ("ode_adaptive", C.c_int), ("ode_abs_tol", C.c_double), ("ode_rel_tol", C.c_double),
         ("alpha_dt", C.c_double),
+        ("instance", C.c_void_p), ("instance_scale", C.c_void_p), ("n_instances", C.c_int),
     ]
 
 

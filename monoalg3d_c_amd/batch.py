@@ -9,9 +9,10 @@ cell carries no flux (the finite-volume operator treats it like a missing mesh
 cell), so the atlas operator is block diagonal, one block per instance, and each
 step of the atlas advances every instance by one step of its own run:
 
-  - the ionic launch covers every cell of the atlas; a per-cell scale field
-    gives each instance its own model parameters (TT2006 GKr/GCaL/GKs, MS2003
-    tau_close) and a per-cell stimulus mask gives each instance its own
+  - the ionic launch covers every cell of the atlas; each cell carries its
+    instance's index (2 bytes) into a table of the instances' model parameters
+    (TT2006 GKr/GCaL/GKs, MS2003 tau_close), which the ionic kernel stages in
+    LDS, and a per-cell stimulus mask gives each instance its own
     stimulated region (x < stim_x_limit in the instance's own frame, the rule of
     stim_if_x_less_than, stimuli_library/stimuli.c);
   - one CG solves all blocks at once.  It stops when the atlas residual r.r is
@@ -36,7 +37,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import MODEL_TT2006, Tissue
+from . import MAX_INSTANCES, MODEL_TT2006, Tissue
 
 
 @dataclass(frozen=True)
@@ -128,13 +129,24 @@ class Layout:
             m[self.cells(i)] = np.tile(hit, inst.ny).astype(np.uint8)
         return m
 
-    def ionic_scale(self) -> np.ndarray:
-        s = np.ones((self.n, 4), np.float32)
+    def instance_ids(self) -> np.ndarray:
+        """Instance of each atlas cell (uint16); gap cells name instance 0
+        (they are inactive and never read it)."""
+        ids = np.zeros(self.n, np.uint16)
+        for i in range(len(self.instances)):
+            ids[self.cells(i)] = i
+        return ids
+
+    def scale_table(self) -> np.ndarray:
+        """The four model factors of each instance, one row per instance."""
+        tab = np.ones((len(self.instances), 4), np.float32)
         for i, inst in enumerate(self.instances):
-            f = np.ones(4, np.float32)
-            f[:len(inst.scale)] = inst.scale
-            s[self.cells(i)] = f
-        return s
+            tab[i, :len(inst.scale)] = inst.scale
+        return tab
+
+    def ionic_scale(self) -> np.ndarray:
+        """The per-cell form of scale_table (n x 4)."""
+        return self.scale_table()[self.instance_ids()]
 
     def split(self, values: np.ndarray) -> list:
         """Per-instance views of an atlas field (leading axis = cells)."""
@@ -155,7 +167,9 @@ class Batch:
     def __init__(self, instances, h=(100.0, 100.0, 100.0), model=MODEL_TT2006, width=None, gap=1, **kw):
         self.instances = list(instances)
         self.layout = Layout(self.instances, width=width, gap=gap)
-        for bad in ("active", "stim_mask", "ionic_scale", "g_cell"):
+        if len(self.instances) > MAX_INSTANCES:
+            raise ValueError(f"{len(self.instances)} instances; one batch holds at most {MAX_INSTANCES}")
+        for bad in ("active", "stim_mask", "ionic_scale", "instance", "instance_scale", "g_cell"):
             if bad in kw:
                 raise ValueError(f"{bad} is set by the batch layout")
         rel = kw.pop("cg_rel_tol", 0.0)
@@ -164,7 +178,8 @@ class Batch:
             a = alpha_of(kw.get("beta", 0.14), kw.get("cm", 1.0), kw.get("dt", 0.02), *h)
             kw["cg_tol"] = rel * rel * min(inst.n for inst in self.instances) * (a * 1.0) ** 2
         self.tissue = Tissue(self.layout.shape, h=h, model=model, active=self.layout.active(),
-                             stim_mask=self.layout.stim_mask(h[0]), ionic_scale=self.layout.ionic_scale(), **kw)
+                             stim_mask=self.layout.stim_mask(h[0]), instance=self.layout.instance_ids(),
+                             instance_scale=self.layout.scale_table(), **kw)
 
     def close(self) -> None:
         self.tissue.close()

@@ -16,6 +16,12 @@ template <typename T> struct IonicLaunch {
     const double *stim_cell;  // optional per-cell stimulus (overrides the x-limit rule)
     const uint8_t *stim_mask; // optional per-cell 0/1: stimulated cells (replaces the x-limit rule)
     const float *scale;     // optional per-cell conductance scalings, 4 per cell
+    // optional per-instance scalings: the instance of each cell (inst) and a
+    // table of 4 factors per instance (inst_scale, ninst rows), staged in LDS
+    // by each block; replaces `scale` when set
+    const uint16_t *inst;
+    const float *inst_scale;
+    int ninst;
     int nx;                 // cells along x (for the centre of a cell)
     double dx;              // cell size along x (um)
     double stim_x_limit;    // cells with centre x < limit are stimulated
@@ -37,6 +43,9 @@ template <typename T> struct IonicLaunch {
 // at a tiny min_dt for a whole call) stops with V = NaN, which the tissue's
 // NaN check reports, instead of looping on.
 constexpr int kMaxAdaptiveSubsteps = 1 << 20;
+
+// largest per-instance table (4 floats per instance, staged in LDS: 64 KB)
+constexpr int kMaxInstances = 4096;
 
 hipError_t launch_ionic_step(int model, const IonicLaunch<double> &a, hipStream_t s);
 hipError_t launch_ionic_step(int model, const IonicLaunch<float> &a, hipStream_t s);

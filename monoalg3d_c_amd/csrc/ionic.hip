@@ -86,8 +86,20 @@ template <typename T> struct RowStore {
 // of scratch spills to ~180 VGPRs without (4.1 -> 2.6 ms at 16M cells).  The
 // FP32 kernels are HBM-bound and keep the loop form: the one-step FP32 ORd
 // kernel runs at 6 waves/SIMD and measured 4% slower than the two-wave one.
+// The per-instance scaling table of a batch, staged in LDS (dynamic shared
+// memory of ninst * 4 floats, given at launch); every thread of the block
+// takes part before any returns.
+template <int MODEL, typename T> __device__ __forceinline__ void stage_instances(const IonicLaunch<T> &a, float *tab) {
+    if constexpr(MODEL == kTT2006 || MODEL == kMS2003) {
+        if(a.inst) {
+            for(int j = threadIdx.x; j < 4 * a.ninst; j += blockDim.x) tab[j] = a.inst_scale[j];
+            __syncthreads();
+        }
+    }
+}
+
 template <int MODEL, typename T, bool ONE>
-__device__ __forceinline__ void ionic_step_body(const IonicLaunch<T> &a) {
+__device__ __forceinline__ void ionic_step_body(const IonicLaunch<T> &a, const float *tab) {
     constexpr int NEQ = ModelTraits<MODEL>::neq;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= a.n) return;
@@ -107,9 +119,15 @@ __device__ __forceinline__ void ionic_step_body(const IonicLaunch<T> &a) {
         return;
     }
     float scale[4] = {1.f, 1.f, 1.f, 1.f};
-    if(a.scale) {
+    if constexpr(MODEL == kTT2006 || MODEL == kMS2003) {
+        if(a.inst) {
+            const int id = a.inst[i];
 #pragma unroll
-        for(int k = 0; k < 4; k++) scale[k] = a.scale[4 * i + k];
+            for(int k = 0; k < 4; k++) scale[k] = tab[4 * id + k];
+        } else if(a.scale) {
+#pragma unroll
+            for(int k = 0; k < 4; k++) scale[k] = a.scale[4 * i + k];
+        }
     }
     for(int k = 0; k < (ONE ? 1 : a.num_steps); k++) {
         y[0] = T(V);
@@ -220,11 +238,15 @@ __global__ __launch_bounds__(256) void k_ionic_adaptive(IonicLaunch<T> a) {
 // wave per SIMD).
 template <int MODEL>
 __global__ __launch_bounds__(256) void k_ionic_step32(IonicLaunch<float> a) {
-    ionic_step_body<MODEL, float, false>(a);
+    extern __shared__ float inst_tab[];
+    stage_instances<MODEL>(a, inst_tab);
+    ionic_step_body<MODEL, float, false>(a, inst_tab);
 }
 template <int MODEL, bool ONE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_ionic_step64(IonicLaunch<double> a) {
-    ionic_step_body<MODEL, double, ONE>(a);
+    extern __shared__ float inst_tab[];
+    stage_instances<MODEL>(a, inst_tab);
+    ionic_step_body<MODEL, double, ONE>(a, inst_tab);
 }
 
 template <int MODEL, typename T>
@@ -281,10 +303,11 @@ template <bool ONE>
 static hipError_t launch_step64(int model, const IonicLaunch<double> &a, hipStream_t s) {
     const int block = 64;
     const unsigned grid = (unsigned)((a.n + block - 1) / block);
+    const size_t lds = a.inst ? sizeof(float) * 4 * (size_t)a.ninst : 0;
     switch(model) {
-    case kTT2006: k_ionic_step64<kTT2006, ONE><<<grid, block, 0, s>>>(a); break;
+    case kTT2006: k_ionic_step64<kTT2006, ONE><<<grid, block, lds, s>>>(a); break;
     case kORd2011: k_ionic_step64<kORd2011, ONE><<<grid, block, 0, s>>>(a); break;
-    case kMS2003: k_ionic_step64<kMS2003, ONE><<<grid, block, 0, s>>>(a); break;
+    case kMS2003: k_ionic_step64<kMS2003, ONE><<<grid, block, lds, s>>>(a); break;
     case kCRN1998: k_ionic_step64<kCRN1998, ONE><<<grid, block, 0, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
@@ -299,10 +322,11 @@ static hipError_t launch_step_t(int model, const IonicLaunch<T> &a, hipStream_t 
     } else {
         const int block = 256;
         const unsigned grid = (unsigned)((a.n + block - 1) / block);
+        const size_t lds = a.inst ? sizeof(float) * 4 * (size_t)a.ninst : 0;
         switch(model) {
-        case kTT2006: k_ionic_step32<kTT2006><<<grid, block, 0, s>>>(a); break;
+        case kTT2006: k_ionic_step32<kTT2006><<<grid, block, lds, s>>>(a); break;
         case kORd2011: k_ionic_step32<kORd2011><<<grid, block, 0, s>>>(a); break;
-        case kMS2003: k_ionic_step32<kMS2003><<<grid, block, 0, s>>>(a); break;
+        case kMS2003: k_ionic_step32<kMS2003><<<grid, block, lds, s>>>(a); break;
         case kCRN1998: k_ionic_step32<kCRN1998><<<grid, block, 0, s>>>(a); break;
         default: return hipErrorInvalidValue;
         }

@@ -83,6 +83,22 @@ static int validate(const mono_tissue_desc *d) {
         mono_set_error("unknown tensor source %d", d->tensor_source);
         return -1;
     }
+    if(d->instance || d->instance_scale) {
+        if(!d->instance || !d->instance_scale || d->n_instances < 1 || d->n_instances > kMaxInstances) {
+            mono_set_error("per-instance scalings need instance, instance_scale and 1 <= n_instances <= %d", kMaxInstances);
+            return -1;
+        }
+        if(d->ionic_scale) {
+            mono_set_error("give per-cell (ionic_scale) or per-instance (instance, instance_scale) scalings, not both");
+            return -1;
+        }
+        const int64_t n = (int64_t)d->nx * d->ny * d->nz;
+        for(int64_t i = 0; i < n; i++)
+            if(d->instance[i] >= d->n_instances) {
+                mono_set_error("cell %lld names instance %d of %d", (long long)i, (int)d->instance[i], d->n_instances);
+                return -1;
+            }
+    }
     return 0;
 }
 
@@ -168,6 +184,8 @@ int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out) {
     t->d.active = nullptr;
     t->d.stim_mask = nullptr;
     t->d.ionic_scale = nullptr;
+    t->d.instance = nullptr;
+    t->d.instance_scale = nullptr;
     const mono_tissue_desc &d = *desc;
     t->n = (int64_t)d.nx * d.ny * d.nz;
     t->neq = model_neq(d.model);
@@ -240,6 +258,15 @@ int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out) {
             return fail();
         }
     }
+    if(d.instance) {
+        const size_t tb = sizeof(float) * 4 * (size_t)d.n_instances;
+        if(hipMalloc(&t->inst, sizeof(uint16_t) * t->n) != hipSuccess || hipMalloc(&t->inst_scale, tb) != hipSuccess ||
+           hipMemcpyAsync(t->inst, d.instance, sizeof(uint16_t) * t->n, hipMemcpyHostToDevice, s) != hipSuccess ||
+           hipMemcpyAsync(t->inst_scale, d.instance_scale, tb, hipMemcpyHostToDevice, s) != hipSuccess) {
+            mono_set_error("instance table upload failed");
+            return fail();
+        }
+    }
 
     // operator
     double *coef27 = nullptr;
@@ -309,7 +336,7 @@ int mono_tissue_destroy(mono_tissue *t) {
     if(t->op.release()) rc = -1;
     if(t->obs.release()) rc = -1;
     if(t->apd.release()) rc = -1;
-    void *bufs[] = {t->sv, t->v, t->b, t->active, t->g, t->tensor, t->stim_mask, t->ionic_scale, t->aux};
+    void *bufs[] = {t->sv, t->v, t->b, t->active, t->g, t->tensor, t->stim_mask, t->ionic_scale, t->aux, t->inst, t->inst_scale};
     for(void *p : bufs)
         if(p && hipFree(p) != hipSuccess) rc = -1;
     for(int k = 0; k < 3; k++)
@@ -409,6 +436,9 @@ static int launch_ionic(mono_tissue *t, double stim_amp) {
         a.active = t->active;
         a.stim_mask = t->stim_mask;
         a.scale = t->ionic_scale;
+        a.inst = t->inst;
+        a.inst_scale = t->inst_scale;
+        a.ninst = d.n_instances;
         a.nx = d.nx;
         a.dx = d.dx;
         a.stim_x_limit = d.stim_x_limit;
@@ -433,6 +463,9 @@ static int launch_ionic(mono_tissue *t, double stim_amp) {
         a.active = t->active;
         a.stim_mask = t->stim_mask;
         a.scale = t->ionic_scale;
+        a.inst = t->inst;
+        a.inst_scale = t->inst_scale;
+        a.ninst = d.n_instances;
         a.nx = d.nx;
         a.dx = d.dx;
         a.stim_x_limit = d.stim_x_limit;

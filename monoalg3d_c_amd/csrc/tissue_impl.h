@@ -22,6 +22,8 @@ struct mono_tissue {
     double *tensor = nullptr;
     uint8_t *stim_mask = nullptr;
     float *ionic_scale = nullptr;
+    uint16_t *inst = nullptr;      // per-cell instance (batches)
+    float *inst_scale = nullptr;   // n_instances x 4 factors
     double *aux = nullptr;  // adaptive ODE solver: dt, time_new, previous_dt rows of `pitch`
     mono::Operator op;
     mono::CgWorkspace cg;

@@ -223,3 +223,31 @@ def test_wrappers_refuse_mis_sized_arrays():
         m.cg_stencil((2, 2, 1), np.zeros((4, 27)), np.ones(3), np.zeros(4))
     with pytest.raises(ValueError, match="alpha and vm"):
         m.update_rhs(np.ones(3), np.ones(4))
+
+
+def test_instance_table_is_validated():
+    """A per-instance scaling table: rows within 1..4096, every cell naming a
+    row of it, and not together with per-cell scalings; the library refuses a
+    descriptor that breaks this before any device work (-1)."""
+    import monoalg3d_c_amd as m
+    from monoalg3d_c_amd import _lib
+    n = 4 * 4 * 1
+    tab = np.ones((3, 4), np.float32)
+    with pytest.raises(ValueError, match="go together"):
+        m.make_desc((4, 4, 1), instance=np.zeros(n, np.uint16))
+    with pytest.raises(ValueError, match="row 3 of a 3-row"):
+        m.make_desc((4, 4, 1), instance=np.full(n, 3, np.uint16), instance_scale=tab)
+    with pytest.raises(ValueError, match="1 to 4096"):
+        m.make_desc((4, 4, 1), instance=np.zeros(n, np.uint16), instance_scale=np.ones((4097, 4), np.float32))
+    with pytest.raises(ValueError, match="not both"):
+        m.make_desc((4, 4, 1), instance=np.zeros(n, np.uint16), instance_scale=tab, ionic_scale=np.ones((n, 4)))
+    L = _lib.lib()
+    d, keep = m.make_desc((4, 4, 1), instance=np.zeros(n, np.uint16), instance_scale=tab)
+    bad = np.full(n, 7, np.uint16)
+    d.instance = bad.ctypes.data
+    t = C.c_void_p()
+    assert L.mono_tissue_create(C.byref(d), C.byref(t)) == -1
+    assert b"names instance 7 of 3" in L.mono_last_error()
+    d.instance, d.n_instances = keep[-2].ctypes.data, 5000
+    assert L.mono_tissue_create(C.byref(d), C.byref(t)) == -1
+    assert b"n_instances" in L.mono_last_error()

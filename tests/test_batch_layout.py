@@ -49,14 +49,18 @@ def test_layout_fields():
     insts = _insts()
     L = Layout(insts)
     m, s = L.stim_mask(100.0), L.ionic_scale()
+    ids, tab = L.instance_ids(), L.scale_table()
+    assert ids.dtype == np.uint16 and tab.shape == (len(insts), 4) and tab.dtype == np.float32
     for i, inst in enumerate(insts):
         c = L.cells(i).reshape(inst.ny, inst.nx)
         lx = np.arange(inst.nx)
         want = ((lx + 0.5) * 100.0 < inst.stim_x_limit).astype(np.uint8)
         assert (m[c] == want[None, :]).all()
         assert (s[c.reshape(-1)] == np.asarray(inst.scale, np.float32)).all()
+        assert (ids[c.reshape(-1)] == i).all()
+        assert (tab[i, :len(inst.scale)] == np.asarray(inst.scale, np.float32)).all()
     gaps = L.active() == 0
-    assert (m[gaps] == 0).all() and (s[gaps] == 1).all()
+    assert (m[gaps] == 0).all()
     v = np.arange(L.n, dtype=float)
     for i, part in enumerate(L.split(v)):
         assert (part == L.cells(i)).all()

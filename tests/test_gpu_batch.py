@@ -94,6 +94,28 @@ def test_batch_matches_oracle_atlas(oracle):
     b.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_instance_table_equals_per_cell_scalings(precision):
+    """The batch's LDS-staged instance table gives the same states, bit for
+    bit, as the same atlas run with the scalings stored per cell."""
+    import monoalg3d_c_amd as m
+    from monoalg3d_c_amd.batch import Batch
+    insts = _mixed()
+    prec = m.FP64 if precision == "fp64" else m.FP32
+    announce(f"batch instance table vs per-cell scalings, {precision}, 10 ms")
+    b = Batch(insts, precision=prec, cg_tol=1e-16, tensor=m.iso_tensor(0.00013))
+    L = b.layout
+    t = m.Tissue(L.shape, h=H, precision=prec, cg_tol=1e-16, tensor=m.iso_tensor(0.00013), active=L.active(),
+                 stim_mask=L.stim_mask(100.0), ionic_scale=L.ionic_scale())
+    b.step(500)
+    t.step(500)
+    act = L.active() == 1
+    assert np.array_equal(b.tissue.state[1:, act], t.state[1:, act])
+    assert np.array_equal(b.tissue.v[act], t.v[act])
+    b.close()
+    t.close()
+
+
 def test_batch_instance_equals_standalone_gpu_run():
     """Each scaled instance of the batch against a stand-alone GPU tissue of
     its own shape with its scalings on every cell (FP64, 20 ms)."""
