@@ -101,7 +101,7 @@ def test_instance_table_equals_per_cell_scalings(precision):
     import monoalg3d_c_amd as m
     from monoalg3d_c_amd.batch import Batch
     insts = _mixed()
-    prec = m.FP64 if precision == "fp64" else m.FP32
+    prec = m.FP64 if precision == "fp64" else m.FP32_REFINE
     announce(f"batch instance table vs per-cell scalings, {precision}, 10 ms")
     b = Batch(insts, precision=prec, cg_tol=1e-16, tensor=m.iso_tensor(0.00013))
     L = b.layout
