@@ -474,6 +474,57 @@ def run_adjoint(args) -> dict:
     }
 
 
+def make_line(args, shape, neq, res, res3, errors, cpu, dropin) -> dict:
+    """The JSON line of the 16M-cell workload from the run results."""
+    n = int(np.prod(shape))
+    head = res.get("fp32") or res["fp64"]
+    mixed = "fp32" in res
+    rf = {k: phase_roofline(r, neq, k == "fp32", 0) for k, r in res.items()}
+    rf3 = {k: phase_roofline(r, neq, k == "fp32", 19 if r["class_operator"] == 0 else 0) for k, r in res3.items()}
+
+    # The dominant kernel is the ionic step: one launch per step, more device
+    # time than any one kernel of the solve.  Its launch time is the ionic
+    # phase, timed with HIP events on the library's stream.
+    hk = "fp32" if mixed else "fp64"
+    d = rf[hk]["ionic"]
+    kname = "k_ionic_step32<1>" if hk == "fp32" else "k_ionic_step64<1, true>"  # ORd 2011 = model 1
+    roofline = {"bound": "hbm", "achieved": round(d["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(d["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname, shape),
+                "kernel": kname, "launch_ms": round(res[hk]["ionic_ms"], 4), "bytes_per_launch": int(d["bytes"])}
+    return {
+        "metric": METRIC,
+        "value": head["rate"],
+        "unit": "cell-timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if mixed else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"cuboid {shape[0]}x{shape[1]}x{shape[2]} cells at 100 um, ORd 2011 endo, "
+                               "anisotropic 0.24/0.045 S/m fibres -60..+60 deg over depth, planar stimulus, "
+                               "CG rel. residual 1e-10, dt 0.02 ms",
+                   "cells": n, "precision_value": "fp32+refine" if mixed else "fp64"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "fp64": res.get("fp64"),
+        "fp32_refine": res.get("fp32"),
+        "speedup_fp32_refine_vs_fp64": (res["fp32"]["rate"] / res["fp64"]["rate"]) if len(res) == 2 else None,
+        "phase_roofline": rf,
+        "fibrotic": None if not res3 else {
+            "workload": "the same cuboid with 10%% of the cells non-conducting (SplitMix64 seed %d, left out as inactive "
+                        "cells; rows stored per cell)" % FIBROSIS_SEED,
+            "active_cells": res3[next(iter(res3))]["cells"], "fp64": res3.get("fp64"), "fp32_refine": res3.get("fp32"),
+            "speedup_fp32_refine_vs_fp64": (res3["fp32"]["rate"] / res3["fp64"]["rate"]) if len(res3) == 2 else None,
+            "phase_roofline": rf3},
+        "errors": errors or None,
+        "dropin_config1": dropin,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -522,8 +573,6 @@ def main() -> int:
         if fib is not None:
             res3[key] = run_precision(shape, prec, args.steps, args.warmup, active=fib)
             print(f"[bench] fibrotic {key} {res3[key]}", file=sys.stderr, flush=True)
-    head = res.get("fp32") or res["fp64"]
-    mixed = "fp32" in res
 
     errors = {}
     if len(res) == 2:
@@ -544,56 +593,13 @@ def main() -> int:
     if args.copy_calib_mb > 0:
         copy_calibration(args.copy_calib_mb)
 
-    rf = {k: phase_roofline(r, neq, k == "fp32", 0) for k, r in res.items()}
-    rf3 = {k: phase_roofline(r, neq, k == "fp32", 19 if r["class_operator"] == 0 else 0) for k, r in res3.items()}
-
-    # The dominant kernel is the ionic step: one launch per step, more device
-    # time than any one kernel of the solve.  Its launch time is the ionic
-    # phase, timed with HIP events on the library's stream.
-    hk = "fp32" if mixed else "fp64"
-    d = rf[hk]["ionic"]
-    kname = "k_ionic_step32<1>" if hk == "fp32" else "k_ionic_step64<1, true>"  # ORd 2011 = model 1
-    roofline = {"bound": "hbm", "achieved": round(d["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(d["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname, shape),
-                "kernel": kname, "launch_ms": round(res[hk]["ionic_ms"], 4), "bytes_per_launch": int(d["bytes"])}
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     if cpu:
         print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
     dropin = None if args.no_dropin else dropin_rates()
     if dropin:
         print(f"[bench] dropin {dropin}", file=sys.stderr, flush=True)
-    out = {
-        "metric": METRIC,
-        "value": head["rate"],
-        "unit": "cell-timesteps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": head["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32" if mixed else "f64",
-        "data": "synthetic",
-        "config": {"workload": f"cuboid {shape[0]}x{shape[1]}x{shape[2]} cells at 100 um, ORd 2011 endo, "
-                               "anisotropic 0.24/0.045 S/m fibres -60..+60 deg over depth, planar stimulus, "
-                               "CG rel. residual 1e-10, dt 0.02 ms",
-                   "cells": n, "precision_value": "fp32+refine" if mixed else "fp64"},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "fp64": res.get("fp64"),
-        "fp32_refine": res.get("fp32"),
-        "speedup_fp32_refine_vs_fp64": (res["fp32"]["rate"] / res["fp64"]["rate"]) if len(res) == 2 else None,
-        "phase_roofline": rf,
-        "fibrotic": None if not res3 else {
-            "workload": "the same cuboid with 10% of the cells non-conducting (SplitMix64 seed %d, left out as inactive "
-                        "cells; rows stored per cell)" % FIBROSIS_SEED,
-            "active_cells": res3[next(iter(res3))]["cells"], "fp64": res3.get("fp64"), "fp32_refine": res3.get("fp32"),
-            "speedup_fp32_refine_vs_fp64": (res3["fp32"]["rate"] / res3["fp64"]["rate"]) if len(res3) == 2 else None,
-            "phase_roofline": rf3},
-        "errors": errors or None,
-        "dropin_config1": dropin,
-    }
+    out = make_line(args, shape, neq, res, res3, errors, cpu, dropin)
     print(json.dumps(out))
     return 0
 
