@@ -282,6 +282,24 @@ int mono_cg_stencil(int nx, int ny, int nz, const double *coef27, const uint8_t 
 int mono_cg_csr(int n, const int64_t *row_ptr, const int32_t *col, const double *val, const double *b,
                 double *x, double tol, int max_its, int jacobi, int *iterations, double *error);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the solve of
+ * mono_cg_csr on operands already on the device (row_ptr, col, val, b and x
+ * are device pointers; x is the warm start and receives the solution) on the
+ * given HIP stream (NULL: the null stream).  Every buffer of the solve lives in
+ * a fixed device area that the library reserves per device when it is loaded
+ * (about 40 MiB, held until the process ends), so n is limited to
+ * MONO_CG_CSR_NOALLOC_MAX_ROWS; larger n is refused.  The operands are checked
+ * on the device (row_ptr starts at 0 and never decreases, columns in [0, n))
+ * before the solve.  The call allocates nothing (no hipMalloc, hipFree, pinned
+ * or host allocation).  Calls that overlap on one device share the area and
+ * race; calls on different devices do not.  Same results as mono_cg_csr, bit
+ * for bit. */
+#define MONO_CG_CSR_NOALLOC_MAX_ROWS (1 << 20)
+int mono_cg_csr_device_noalloc(int n, const int64_t *row_ptr, const int32_t *col, const double *val, const double *b,
+                               double *x, double tol, int max_its, int jacobi, int *iterations, double *error,
+                               void *stream);
+
 /* A CSR operator resident on the device between solves.
  *   mono_csr_create   INIT_LINEAR_SYSTEM (linear_system_solver_config.h:18) as
  *                     init_gpu_conjugate_gradient does it (gpu_solvers_cublas_12.c:46-119):

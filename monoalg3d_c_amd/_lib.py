@@ -28,6 +28,7 @@ EXPORTED = (
     "mono_sensor_gradient", "mono_csr_create", "mono_csr_num_rows", "mono_csr_set_alpha", "mono_csr_update_rhs",
     "mono_csr_get_rhs", "mono_csr_solve", "mono_csr_destroy", "mono_update_rhs", "mono_ode_gpu_alloc",
     "mono_ode_gpu_solve", "mono_device_free", "mono_tissue_apd_setup", "mono_tissue_apd_sample", "mono_tissue_apd_get",
+    "mono_cg_csr_device_noalloc",
 )
 
 

@@ -16,6 +16,12 @@
 
 namespace mono {
 
+// Per-block partial sums: at most kMaxBlocks blocks (or runs) per reduction
+// kernel, at most kMaxRed sums each, stored sum-major (sum k of block b at
+// k * kMaxBlocks + b).
+constexpr int kMaxBlocks = 8192;
+constexpr int kMaxRed = 3;
+
 struct CsrMatrix {
     int n = 0;
     int64_t nnz = 0;

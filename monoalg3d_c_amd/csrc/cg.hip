@@ -28,8 +28,6 @@
 namespace mono {
 
 constexpr int kBlock = 256;
-constexpr int kMaxBlocks = 8192;
-constexpr int kMaxRed = 3;
 constexpr int kElemBlocks = 1024;  // grid of the elementwise kernels
 
 __host__ __device__ __forceinline__ int pos_cls(int i, int len) { return i == 0 ? 0 : (i == len - 1 ? 2 : 1); }

@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "../../include/monoalg_hip.h"
@@ -55,7 +56,146 @@ static int check_csr(const char *who, int n, const int64_t *row_ptr, const int32
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// mono_cg_csr_device_noalloc: the CSR solve of mono_cg_csr on operands the
+// caller already holds on the device, with every buffer of the solve in one
+// fixed area of device memory that the library reserves at load time, so that
+// a call allocates nothing.  Layout (doubles): diagonal, r, p, q, z (n each,
+// n <= kNoAllocMaxRows), two partial-sum areas (kMaxBlocks * kMaxRed each),
+// the scalar record, then the operand check's verdict.  Each device holds its
+// own copy of the area; concurrent calls on one device share it and race.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocMaxRows = MONO_CG_CSR_NOALLOC_MAX_ROWS;
+constexpr size_t kNoAllocVecBytes = sizeof(double) * (size_t)kNoAllocMaxRows;
+constexpr size_t kNoAllocPartBytes = sizeof(double) * (size_t)kMaxBlocks * kMaxRed;
+constexpr size_t kNoAllocScOff = 5 * kNoAllocVecBytes + 2 * kNoAllocPartBytes;
+constexpr size_t kNoAllocCheckOff = kNoAllocScOff + 256;
+constexpr size_t kNoAllocBytes = kNoAllocCheckOff + 256;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static void *g_scratch_ptr[kNoAllocMaxDevices];  // device address of scratch, per device
+static CgScalars g_noalloc_host_sc[kNoAllocMaxDevices];
+static std::mutex g_scratch_mu;
+
+// The device address of this device's scratch (taken once per device, with
+// that device current; the array's name is not a device address in host code).
+static unsigned char *scratch_on_current_device(const char *who) {
+    int dev = -1;
+    if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices) {
+        mono_set_error("%s: no current HIP device", who);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if(!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if(e != hipSuccess || !ptr) {
+            mono_set_error("%s: hipGetSymbolAddress(scratch): %s", who, hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    return (unsigned char *)g_scratch_ptr[dev];
+}
+
+// Device-side operand check (the host cannot read device operands without a
+// staging buffer): bit 0 row_ptr[0] != 0, bit 1 a decreasing row_ptr, bit 2
+// row_ptr[n] < 0, bit 3 a column outside [0, n), ORed into verdict[0], which
+// the caller clears first (with a kernel, not a memset).
+__global__ void k_clear_verdict(int *verdict) {
+    if(threadIdx.x == 0) verdict[0] = 0;
+}
+__global__ void k_csr_check_rows(int n, const int64_t *__restrict__ rp, int *verdict) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i > n) return;
+    int bad = 0;
+    if(i == 0 && rp[0] != 0) bad |= 1;
+    if(i < n && rp[i + 1] < rp[i]) bad |= 2;
+    if(i == n && rp[n] < 0) bad |= 4;
+    if(bad) atomicOr(verdict, bad);
+}
+__global__ void k_csr_check_cols(int n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int *verdict) {
+    const int64_t nnz = rp[n];
+    for(int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x)
+        if(col[k] < 0 || col[k] >= n) {
+            atomicOr(verdict, 8);
+            return;
+        }
+}
+
 extern "C" {
+
+int mono_cg_csr_device_noalloc(int n, const int64_t *row_ptr, const int32_t *col, const double *val, const double *b,
+                               double *x, double tol, int max_its, int jacobi, int *iterations, double *error,
+                               void *stream) {
+    const char *who = "mono_cg_csr_device_noalloc";
+    if(n <= 0 || !row_ptr || !col || !val || !b || !x) {
+        mono_set_error("%s: bad arguments", who);
+        return -1;
+    }
+    if((int64_t)n > kNoAllocMaxRows) {
+        mono_set_error("%s: %d rows exceed the fixed workspace's maximum of %lld rows", who, n, (long long)kNoAllocMaxRows);
+        return -1;
+    }
+    unsigned char *base = scratch_on_current_device(who);
+    if(!base) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    const int dev = [] {
+        int d = 0;
+        (void)hipGetDevice(&d);
+        return d;
+    }();
+    // operand check before any kernel indexes col/val through row_ptr
+    int *verdict = (int *)(base + kNoAllocCheckOff);
+    k_clear_verdict<<<1, 64, 0, s>>>(verdict);
+    k_csr_check_rows<<<(unsigned)((n + 1 + 255) / 256), 256, 0, s>>>(n, row_ptr, verdict);
+    MONO_CHECK(hipGetLastError());
+    int flags = 0;
+    MONO_CHECK(hipMemcpyAsync(&flags, verdict, sizeof(int), hipMemcpyDeviceToHost, s));
+    MONO_CHECK(hipStreamSynchronize(s));
+    if(flags) {
+        mono_set_error("%s: malformed row_ptr (%s%s%s)", who, flags & 1 ? "row_ptr[0] != 0 " : "",
+                       flags & 2 ? "decreasing entries " : "", flags & 4 ? "row_ptr[n] < 0" : "");
+        return -1;
+    }
+    k_csr_check_cols<<<1024, 256, 0, s>>>(n, row_ptr, col, verdict);
+    MONO_CHECK(hipGetLastError());
+    MONO_CHECK(hipMemcpyAsync(&flags, verdict, sizeof(int), hipMemcpyDeviceToHost, s));
+    MONO_CHECK(hipStreamSynchronize(s));
+    if(flags) {
+        mono_set_error("%s: column index out of range", who);
+        return -1;
+    }
+    double *v = (double *)base;
+    CsrMatrix M;
+    M.n = n;
+    M.row_ptr = row_ptr;
+    M.col = col;
+    M.val = val;
+    M.diag = v;
+    MONO_CHECK(launch_csr_diag(M, v, s));
+    CsrBuffers w;
+    w.r = v + kNoAllocMaxRows;
+    w.p = v + 2 * kNoAllocMaxRows;
+    w.q = v + 3 * kNoAllocMaxRows;
+    w.z = v + 4 * kNoAllocMaxRows;
+    w.P0 = (double *)(base + 5 * kNoAllocVecBytes);
+    w.P1 = (double *)(base + 5 * kNoAllocVecBytes + kNoAllocPartBytes);
+    w.sc = (CgScalars *)(base + kNoAllocScOff);
+    w.host_sc = &g_noalloc_host_sc[dev];
+    CgConfig cfg;
+    cfg.tol = tol;
+    cfg.max_its = max_its;
+    cfg.jacobi = jacobi;
+    CgStats st{};
+    if(csr_solve(M, b, x, cfg, &st, s, w)) return -1;
+    MONO_CHECK(hipStreamSynchronize(s));
+    if(iterations) *iterations = st.iterations;
+    if(error) *error = st.error;
+    return 0;
+}
 
 int mono_csr_destroy(mono_csr *A) {
     if(!A) return 0;
