@@ -254,19 +254,21 @@ constexpr int kNLD = (kPL + kBlock - 1) / kBlock;
 constexpr int kPLS = kNLD * kBlock;  // LDS plane stride: every lane of every load set has a slot
 constexpr int kRPT = kTX * kTY / kBlock;  // rows per thread
 
-// Work split of the plane-loop kernels.  The box is cut into nslab z slabs of
-// L planes and its xy plane into ntiles tiles; a run is (slab, group of tpr
-// consecutive tiles), numbered slab-major, so the runs of one slab are
-// neighbouring tiles.  Runs are a fixed function of the box and the device's
-// CU count: every run writes its own partial sums, and the next kernel adds
-// them in run order, so the reductions (and the solver's results) do not
-// depend on how many blocks the launch has.  Each block takes rpb consecutive
-// runs; blocks of one XCD take adjacent ones, so the tiles in flight on an XCD
-// at any time are neighbours at about the same depth and their shared halo
-// rows come from its L2.
+// Work split of the plane-loop kernels.  The items are (xy tile, z plane)
+// pairs in tile-major order (a tile's planes are consecutive); run r is the
+// contiguous item range [r * per, (r + 1) * per), so a run walks down one
+// tile's planes and possibly continues on the next tile.  There are as many
+// runs as the kernel has resident blocks (at least kMinRun items each), one
+// per block, so all blocks finish together.  The resident count comes from
+// the occupancy query for the kernel on this device, which is the same in
+// every process: the runs, and so the order in which the next kernel adds
+// their partial sums, are a fixed function of the box, the kernel and the
+// device model, and a restarted run repeats the reductions bit for bit.
+// Blocks of one XCD take adjacent runs, so the tiles in flight on an XCD are
+// neighbours and their shared halo rows come from its L2.
 struct TileGeom {
     int ntx, nty, nblocks;
-    int ntiles, L, nslab, tpr, nruns, rpb;
+    int ntiles, nruns, per;
     int lo[3], hi[3];      // interior range of each axis (extent 1: its cell; extent 2: empty)
 };
 
@@ -517,19 +519,23 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
 
 // Call f(tile, z0, z1) for every tile and plane range of run r.
 template <class F> __device__ __forceinline__ void for_run(const TileGeom &g, int nz, int r, F &&f) {
-    const int slab = r / (g.nruns / g.nslab), grp = r % (g.nruns / g.nslab);
-    const int z0 = slab * g.L, z1 = z0 + g.L < nz ? z0 + g.L : nz;
-    const int t0 = grp * g.tpr, t1 = t0 + g.tpr < g.ntiles ? t0 + g.tpr : g.ntiles;
-    for(int t = t0; t < t1; t++) f(t, z0, z1);
+    const int64_t nitems = (int64_t)g.ntiles * nz;
+    int64_t w = (int64_t)r * g.per;
+    const int64_t end = w + g.per < nitems ? w + g.per : nitems;
+    while(w < end) {
+        const int t = (int)(w / nz), z0 = (int)(w % nz);
+        const int z1 = (int64_t)nz - z0 < end - w ? nz : z0 + (int)(end - w);
+        f(t, z0, z1);
+        w += z1 - z0;
+    }
 }
 
-// This block's runs: each run's partial sums go to its own slot of Pout.
+// This block's run: its partial sums go to the run's own slot of Pout.
 template <typename TV, typename TC, int CM, int PAT, bool FUSE, class Epi>
 __device__ __forceinline__ void tile_sweep(const OpView<TC> &op, const TileGeom &g, const TV *__restrict__ x, const TV *__restrict__ rz,
                                            TV beta, TV *__restrict__ p_out, const Epi &epi, double *Pout) {
-    const int Lb = xcd_chunk(blockIdx.x, gridDim.x);
-    const int r0 = Lb * g.rpb, r1 = r0 + g.rpb < g.nruns ? r0 + g.rpb : g.nruns;
-    for(int r = r0; r < r1; r++) {
+    const int r = xcd_chunk(blockIdx.x, gridDim.x);
+    if(r < g.nruns) {
         double v[Epi::NR];
 #pragma unroll
         for(int j = 0; j < Epi::NR; j++) v[j] = 0.0;
@@ -545,11 +551,10 @@ __device__ __forceinline__ void tile_sweep(const OpView<TC> &op, const TileGeom 
 template <typename TV, typename TC>
 __device__ __forceinline__ void defer_apply(const OpView<TC> &op, const TileGeom &g, TV *__restrict__ x, const TV *__restrict__ p,
                                             TV alpha) {
-    const int Lb = xcd_chunk(blockIdx.x, gridDim.x);
-    const int r0 = Lb * g.rpb, r1 = r0 + g.rpb < g.nruns ? r0 + g.rpb : g.nruns;
+    const int r = xcd_chunk(blockIdx.x, gridDim.x);
     const int64_t sz = (int64_t)op.nx * op.ny;
     const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
-    for(int r = r0; r < r1; r++)
+    if(r < g.nruns)
         for_run(g, op.nz, r, [&](int t, int z0, int z1) {
             const int gx = (t % g.ntx) * kTX + tx, y0 = (t / g.ntx) * kTY;
             if(gx >= op.nx) return;
@@ -563,16 +568,6 @@ __device__ __forceinline__ void defer_apply(const OpView<TC> &op, const TileGeom
                 }
             }
         });
-}
-
-// Calibration (mode -2): every block records when it started and stays for
-// `spin` ticks of the 100 MHz wall clock, so the host can count how many
-// blocks of this kernel are resident at once.
-__device__ __forceinline__ void residency_probe(int spin, double *out) {
-    const unsigned long long t0 = wall_clock64();
-    if(threadIdx.x == 0) out[blockIdx.x] = (double)t0;
-    while(wall_clock64() - t0 < (unsigned long long)spin) {
-    }
 }
 
 // One-plane domains (nz == 1) with 7-point rows (5 in the plane): one wave per
@@ -706,10 +701,6 @@ template <typename TV, typename TC, int CM, int PAT, bool FUSE, class Epi>
 __global__ __launch_bounds__(kBlock) void k_stencil(OpView<TC> op, TileGeom g, const TV *__restrict__ x, const TV *__restrict__ rz,
                                                     TV *__restrict__ p_out, Epi epi, CgSetup cs, int mode, int k, const double *Pin,
                                                     int nprev, double *Pout, CgScalars *sc, CgScalars *so) {
-    if(mode == -2) {
-        residency_probe(k, Pout);
-        return;
-    }
     double betad = 0.0;
     bool now = false;
     Epi e = epi;
@@ -1041,14 +1032,11 @@ template <typename TC, class F> static void with_op(const OpView<TC> &op, F &&f)
     });
 }
 
-// Runs of the plane-loop kernels (see TileGeom), a function of the box and the
-// CU count only: about kRunsPerCU runs per CU (MONO_RUNS_PER_CU overrides),
-// slabs of at least kMinRun planes, at most kMaxBlocks runs (the partial-sum
-// slots).  The launch has one block per run, or as many blocks as are resident
-// at once when fewer (`slots`, measured), each then taking consecutive runs.
+// Runs of the plane-loop kernels (see TileGeom): one per resident block (at
+// most kMaxBlocks, the partial-sum slots; at least kMinRun items each), in
+// whole rounds of the XCD dispatch order.
 constexpr int kMinRun = 8;
-constexpr int kRunsPerCU = 6;
-template <typename TC> static TileGeom make_geom(const OpView<TC> &op, int slots, int cus) {
+template <typename TC> static TileGeom make_geom(const OpView<TC> &op, int slots) {
     TileGeom g{};
     const int L[3] = {op.nx, op.ny, op.nz};
     for(int a = 0; a < 3; a++) {
@@ -1058,46 +1046,16 @@ template <typename TC> static TileGeom make_geom(const OpView<TC> &op, int slots
     g.ntx = (op.nx + kTX - 1) / kTX;
     g.nty = (op.ny + kTY - 1) / kTY;
     g.ntiles = g.ntx * g.nty;
-    static int per_cu = 0;
-    if(!per_cu) {
-        const char *e = getenv("MONO_RUNS_PER_CU");
-        per_cu = e && atoi(e) > 0 ? atoi(e) : kRunsPerCU;
-    }
-    const int64_t target = (int64_t)(cus > 0 ? cus : 1) * per_cu;
-    int64_t nslab = target / g.ntiles;
-    if(nslab < 1) nslab = 1;
-    int64_t len = (op.nz + nslab - 1) / nslab;
-    if(len < kMinRun) len = op.nz < kMinRun ? op.nz : kMinRun;
-    nslab = (op.nz + len - 1) / len;
-    g.L = (int)len;
-    g.nslab = (int)nslab;
-    // tiles per run: one, unless the partial-sum slots run out
-    int64_t groups = g.ntiles;
-    g.tpr = 1;
-    while(groups * nslab > kMaxBlocks) {
-        g.tpr++;
-        groups = (g.ntiles + g.tpr - 1) / g.tpr;
-    }
-    g.nruns = (int)(groups * nslab);
-    // blocks: one per run when that many are resident, in whole rounds of the
-    // XCD dispatch order
-    int64_t nb = g.nruns;
-    if(nb >= 16) nb = (nb + 7) / 8 * 8;
-    if(nb > slots) nb = slots >= 16 ? slots / 8 * 8 : slots;
+    const int64_t nitems = (int64_t)g.ntiles * op.nz;
+    int64_t nb = (nitems + kMinRun - 1) / kMinRun;
+    if(nb > slots) nb = slots;
+    if(nb > kMaxBlocks) nb = kMaxBlocks;
+    if(nb >= 16) nb = nb / 8 * 8;
     if(nb < 1) nb = 1;
-    g.rpb = (int)((g.nruns + nb - 1) / nb);
+    g.per = (int)((nitems + nb - 1) / nb);
+    g.nruns = (int)((nitems + g.per - 1) / g.per);
     g.nblocks = (int)nb;
     return g;
-}
-
-// Count of blocks that started within 10 us of the first in a calibration
-// launch (start times in 100 MHz ticks).
-static int count_resident(const double *start, int n) {
-    double t0 = start[0];
-    for(int i = 1; i < n; i++) t0 = start[i] < t0 ? start[i] : t0;
-    int c = 0;
-    for(int i = 0; i < n; i++) c += start[i] < t0 + 1000.0;
-    return c;
 }
 
 static int check_pattern(int pat) {
@@ -1244,10 +1202,6 @@ static int poll(CgScalars *dev, CgScalars *host, hipStream_t s) {
 // Launch one stencil kernel instance over its geometry; returns the number of
 // partial sums it leaves (its runs; the strip kernel: its blocks), or -1 on a
 // HIP error.
-// On first use per device the instance is calibrated: a launch in mode -2 with
-// more blocks than fit, each staying 30 us, shows how many are resident at
-// once (the occupancy query alone over-counts kernels with many scalar
-// registers); the start times go through Pout, which holds nothing live yet.
 template <typename TV, typename TC, int C, int P, bool FUSE, class Epi>
 static int launch_stencil(const OpView<TC> &op, const TV *x, const TV *rz, TV *p_out, const Epi &epi, const CgSetup &cs, int mode,
                           int k, const double *Pin, int nprev, double *Pout, CgScalars *sc, CgScalars *so, hipStream_t s) {
@@ -1269,22 +1223,12 @@ static int launch_stencil(const OpView<TC> &op, const TV *x, const TV *rz, TV *p
         int cus = 0, per_cu = 0;
         MONO_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         MONO_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(K), kBlock, 0));
-        int n = cus * (per_cu + 1);
-        if(n > kMaxBlocks * kMaxRed) n = kMaxBlocks * kMaxRed;
-        if(n < 1) n = 1;
-        K<<<n, kBlock, 0, s>>>(op, TileGeom{}, x, rz, p_out, epi, cs, -2, 3000, nullptr, 0, Pout, nullptr, nullptr);
-        MONO_CHECK(hipGetLastError());
-        std::vector<double> start(n);
-        MONO_CHECK(hipMemcpyAsync(start.data(), Pout, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-        MONO_CHECK(hipStreamSynchronize(s));
-        slots[dev] = count_resident(start.data(), n);
+        slots[dev] = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
         if(getenv("MONO_TRACE_SLOTS"))
-            fprintf(stderr, "[mono] stencil<%s,%d,%d,%d,NR=%d>: %d blocks resident (occupancy query %d x %d CUs)\n",
-                    sizeof(TV) == 8 ? "f64" : "f32", C, P, (int)FUSE, Epi::NR, slots[dev], per_cu, cus);
+            fprintf(stderr, "[mono] stencil<%s,%d,%d,%d,NR=%d>: %d resident blocks (%d x %d CUs)\n", sizeof(TV) == 8 ? "f64" : "f32",
+                    C, P, (int)FUSE, Epi::NR, slots[dev], per_cu, cus);
     }
-    static int cus[64] = {};
-    if(!cus[dev]) MONO_CHECK(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
-    const TileGeom g = make_geom(op, slots[dev], cus[dev]);
+    const TileGeom g = make_geom(op, slots[dev]);
     K<<<g.nblocks, kBlock, 0, s>>>(op, g, x, rz, p_out, epi, cs, mode, k, Pin, nprev, Pout, sc, so);
     return g.nruns;
 }
