@@ -56,10 +56,14 @@ __global__ __launch_bounds__(256) void k_scatter_add(double *lam, const int64_t 
     if(i < m) lam[idx[i]] += val[i];
 }
 
-// state row 1 (h) of the tissue in FP64
-template <typename T> __global__ __launch_bounds__(256) void k_copy_row(const T *src, double *dst, int64_t n) {
+// V and h (state row 1, FP64) of the tissue in one launch
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_vh(const double *v, const T *h, double *vdst, double *hdst, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(i < n) dst[i] = (double)src[i];
+    if(i < n) {
+        vdst[i] = v[i];
+        hdst[i] = (double)h[i];
+    }
 }
 
 // One backward step per cell, given w = A^-1 lambda_{n+1} (and the previous
@@ -207,13 +211,13 @@ int restore(mono_tissue *t, const Checkpoint &c, size_t sv_bytes) {
     return 0;
 }
 
-// h (state row 1) of the tissue into dst, FP64
-int copy_h(mono_tissue *t, double *dst) {
+// V and h of the tissue into vdst, hdst (FP64)
+int copy_vh(mono_tissue *t, double *vdst, double *hdst) {
     const unsigned grid = (unsigned)((t->n + 255) / 256);
     if(t->d.precision == MONO_FP64)
-        k_copy_row<double><<<grid, 256, 0, t->stream>>>((const double *)t->sv + t->pitch, dst, t->n);
+        k_copy_vh<double><<<grid, 256, 0, t->stream>>>(t->v, (const double *)t->sv + t->pitch, vdst, hdst, t->n);
     else
-        k_copy_row<float><<<grid, 256, 0, t->stream>>>((const float *)t->sv + t->pitch, dst, t->n);
+        k_copy_vh<float><<<grid, 256, 0, t->stream>>>(t->v, (const float *)t->sv + t->pitch, vdst, hdst, t->n);
     MONO_CHECK(hipGetLastError());
     return 0;
 }
@@ -270,10 +274,7 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
             return -1;
         }
     }
-    if(want_grad && store_all) {
-        MONO_CHECK(hipMemcpyAsync(Vseg.p, t->v, nb, hipMemcpyDeviceToDevice, t->stream));
-        if(copy_h(t, Hseg.as<double>())) return -1;
-    }
+    if(want_grad && store_all && copy_vh(t, Vseg.as<double>(), Hseg.as<double>())) return -1;
     for(int k = 0; k < nsteps; k++) {
         if(!store_all && k % K == 0 && save(t, ck[k / K], sv_bytes)) return -1;
         if(k % record_every == 0 && record(t, idx, nsens, tmp, &tr[(size_t)(k / record_every) * nsens])) return -1;
@@ -283,10 +284,8 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
             mono_set_error("mono_sensor_gradient: the forward run stopped on a NaN at step %d", k);
             return -1;
         }
-        if(want_grad && store_all) {
-            MONO_CHECK(hipMemcpyAsync(Vseg.as<double>() + (size_t)(k + 1) * n, t->v, nb, hipMemcpyDeviceToDevice, t->stream));
-            if(copy_h(t, Hseg.as<double>() + (size_t)(k + 1) * n)) return -1;
-        }
+        if(want_grad && store_all && copy_vh(t, Vseg.as<double>() + (size_t)(k + 1) * n, Hseg.as<double>() + (size_t)(k + 1) * n))
+            return -1;
     }
     if(nsteps % record_every == 0 && record(t, idx, nsens, tmp, &tr[(size_t)(nsteps / record_every) * nsens])) return -1;
     if(traces) memcpy(traces, tr.data(), sizeof(double) * tr.size());
@@ -347,12 +346,10 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
         if(!store_all) {
             // recompute the segment from its checkpoint, keeping its states
             if(restore(t, ck[seg], sv_bytes)) return -1;
-            MONO_CHECK(hipMemcpyAsync(VS, t->v, nb, hipMemcpyDeviceToDevice, t->stream));
-            if(copy_h(t, HS)) return -1;
+            if(copy_vh(t, VS, HS)) return -1;
             for(int k = k0; k < k1; k++) {
                 if(mono_tissue_step(t, 1) < 0) return -1;
-                MONO_CHECK(hipMemcpyAsync(VS + (size_t)(k - k0 + 1) * n, t->v, nb, hipMemcpyDeviceToDevice, t->stream));
-                if(copy_h(t, HS + (size_t)(k - k0 + 1) * n)) return -1;
+                if(copy_vh(t, VS + (size_t)(k - k0 + 1) * n, HS + (size_t)(k - k0 + 1) * n)) return -1;
             }
         }
         auto c1 = now();

@@ -133,6 +133,10 @@ class CgWorkspace {
     // Enqueue `enqueue`'s launches on s, through a recorded graph when the key
     // has been seen before; returns 0 or -1 (HIP error).
     template <class F> int run_prefix(const PrefixKey &key, hipStream_t s, int &state, F &&enqueue);
+
+  private:
+    int fp64_launches(const OpView<double> &op, const double *b, double *x, const CgConfig &cfg, int k0, int count, int &nres,
+                      hipStream_t s);
 };
 
 constexpr int kMaxBlindSweeps = 8;

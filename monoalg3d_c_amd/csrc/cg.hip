@@ -28,6 +28,9 @@
 namespace mono {
 
 constexpr int kBlock = 256;
+#ifndef MONO_SLAB_RUNS
+#define MONO_SLAB_RUNS 1
+#endif
 constexpr int kElemBlocks = 1024;  // grid of the elementwise kernels
 
 __host__ __device__ __forceinline__ int pos_cls(int i, int len) { return i == 0 ? 0 : (i == len - 1 ? 2 : 1); }
@@ -269,6 +272,7 @@ constexpr int kRPT = kTX * kTY / kBlock;  // rows per thread
 struct TileGeom {
     int ntx, nty, nblocks;
     int ntiles, nruns, per;
+    int L;                 // slab mode: planes per slab (0: runs are item ranges)
     int lo[3], hi[3];      // interior range of each axis (extent 1: its cell; extent 2: empty)
 };
 
@@ -519,6 +523,12 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
 
 // Call f(tile, z0, z1) for every tile and plane range of run r.
 template <class F> __device__ __forceinline__ void for_run(const TileGeom &g, int nz, int r, F &&f) {
+    if(g.L > 0) {  // slab mode: run r is tile r % ntiles over slab r / ntiles
+        const int slab = r / g.ntiles, t = r % g.ntiles;
+        const int z0 = slab * g.L, z1 = z0 + g.L < nz ? z0 + g.L : nz;
+        f(t, z0, z1);
+        return;
+    }
     const int64_t nitems = (int64_t)g.ntiles * nz;
     int64_t w = (int64_t)r * g.per;
     const int64_t end = w + g.per < nitems ? w + g.per : nitems;
@@ -1046,6 +1056,27 @@ template <typename TC> static TileGeom make_geom(const OpView<TC> &op, int slots
     g.ntx = (op.nx + kTX - 1) / kTX;
     g.nty = (op.ny + kTY - 1) / kTY;
     g.ntiles = g.ntx * g.nty;
+#if MONO_SLAB_RUNS
+    // slab mode: every tile of the plane in each of nslab z slabs, one run per
+    // block, all runs resident at once; the runs of a slab are consecutive,
+    // so an XCD's blocks march neighbouring tiles through the same planes
+    // together and their shared halo lines are fetched from HBM once
+    if(g.ntiles <= slots) {
+        int64_t nslab = slots / g.ntiles;
+        if(nslab > op.nz) nslab = op.nz;
+        g.L = (int)((op.nz + nslab - 1) / nslab);
+        nslab = (op.nz + g.L - 1) / g.L;
+        g.nruns = (int)(g.ntiles * nslab);
+        if(g.nruns <= kMaxBlocks) {
+            g.per = 0;
+            int64_t nb = g.nruns;
+            if(nb >= 16) nb = (nb + 7) / 8 * 8;
+            g.nblocks = (int)nb;
+            return g;
+        }
+        g.L = 0;
+    }
+#endif
     const int64_t nitems = (int64_t)g.ntiles * op.nz;
     int64_t nb = (nitems + kMinRun - 1) / kMinRun;
     if(nb > slots) nb = slots;
@@ -1233,9 +1264,11 @@ static int launch_stencil(const OpView<TC> &op, const TV *x, const TV *rz, TV *p
     return g.nruns;
 }
 
-int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x, const CgConfig &cfg, CgStats *st, hipStream_t s) {
+// The launches of an FP64 solve: the residual (k0 == 0; its partial count goes
+// to nres) and iterations k0 .. k0 + count - 1.
+int CgWorkspace::fp64_launches(const OpView<double> &op, const double *b, double *x, const CgConfig &cfg, int k0, int count,
+                               int &nres, hipStream_t s) {
     const int grid = grid_for(op.n);
-    if(check_pattern(op.pat)) return -1;
     const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, cfg.jacobi, cfg.max_outer, 0};
     double *pb[2] = {p, p2};
     const double *rz = cfg.jacobi ? z : r;
@@ -1244,8 +1277,7 @@ int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x
     // sweep, which reads p anyway (72 instead of 80 bytes per cell and
     // iteration); the sweep that ends the solve applies the last one
     const bool defer = op.nz > 1;
-    int nres = 0;
-    auto residual = [&]() -> int {
+    if(k0 == 0) {
         // residual: r = b - A x, p = r (or z), partials r.r, r.z, b.b -> P0
         with_op(op, [&](auto C_, auto P_) {
             constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
@@ -1258,30 +1290,33 @@ int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x
         });
         if(nres < 0) return -1;
         MONO_CHECK(hipGetLastError());
-        return 0;
-    };
-    auto iterations = [&](int k0, int count) -> int {
-        for(int k = k0; k < k0 + count; k++) {
-            double *pin = pb[k & 1], *pout = pb[(k + 1) & 1];
-            // iteration k: stopping test and beta from P0; p = rz + beta p; q = A p; p.q -> P1
-            int nst = 0;
-            with_op(op, [&](auto C_, auto P_) {
-                constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
-                if(defer)
-                    nst = launch_stencil<double, double, C, P, true>(op, pin, rz, pout, EpiSpmvX<double>{q, x, pin}, cs,
-                                                                     k == 0 ? 0 : 1, k, P0, k == 0 ? nres : grid, P1, sc, nullptr, s);
-                else
-                    nst = launch_stencil<double, double, C, P, true>(op, pin, rz, pout, EpiSpmv<double>{q}, cs, k == 0 ? 0 : 1, k,
-                                                                     P0, k == 0 ? nres : grid, P1, sc, nullptr, s);
-            });
-            if(nst < 0) return -1;
-            // alpha from P1; x += alpha p (unless deferred); r -= alpha q; r.r, r.z -> P0
-            k_update<double, double, double><<<grid, kBlock, 0, s>>>(op, x, pout, q, r, z, cfg.jacobi, k, P1, nst, P0, sc, 0,
-                                                                     defer ? 1 : 0);
-        }
-        MONO_CHECK(hipGetLastError());
-        return 0;
-    };
+    }
+    for(int k = k0; k < k0 + count; k++) {
+        double *pin = pb[k & 1], *pout = pb[(k + 1) & 1];
+        // iteration k: stopping test and beta from P0; p = rz + beta p; q = A p; p.q -> P1
+        int nst = 0;
+        with_op(op, [&](auto C_, auto P_) {
+            constexpr int C = decltype(C_)::value, P = decltype(P_)::value;
+            if(defer)
+                nst = launch_stencil<double, double, C, P, true>(op, pin, rz, pout, EpiSpmvX<double>{q, x, pin}, cs, k == 0 ? 0 : 1, k,
+                                                                 P0, k == 0 ? nres : grid, P1, sc, nullptr, s);
+            else
+                nst = launch_stencil<double, double, C, P, true>(op, pin, rz, pout, EpiSpmv<double>{q}, cs, k == 0 ? 0 : 1, k, P0,
+                                                                 k == 0 ? nres : grid, P1, sc, nullptr, s);
+        });
+        if(nst < 0) return -1;
+        // alpha from P1; x += alpha p (unless deferred); r -= alpha q; r.r, r.z -> P0
+        k_update<double, double, double><<<grid, kBlock, 0, s>>>(op, x, pout, q, r, z, cfg.jacobi, k, P1, nst, P0, sc, 0,
+                                                                 defer ? 1 : 0);
+    }
+    MONO_CHECK(hipGetLastError());
+    return 0;
+}
+
+int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x, const CgConfig &cfg, CgStats *st, hipStream_t s) {
+    if(check_pattern(op.pat)) return -1;
+    int nres = 0;
+    auto iterations = [&](int k0, int count) -> int { return fp64_launches(op, b, x, cfg, k0, count, nres, s); };
     // the last solve's iteration count includes the kernel that detected
     // convergence: enqueue that many, so one poll usually sees the solve done
     int chunk = cfg.poll_chunk > 0 ? cfg.poll_chunk : (last_iters > 2 ? last_iters : 2);
@@ -1293,7 +1328,7 @@ int CgWorkspace::solve_fp64(const OpView<double> &op, const double *b, double *x
     key.x = x;
     key.cfg = cfg;
     int no_state = 0;
-    if(run_prefix(key, s, no_state, [&]() -> int { return residual() ? -1 : iterations(0, chunk); })) return -1;
+    if(run_prefix(key, s, no_state, [&]() -> int { return iterations(0, chunk); })) return -1;
     int enq = chunk;
     for(;;) {
         if(poll(sc, host_sc, s)) return -1;
