@@ -498,7 +498,11 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
     // class rows unconditionally, so that their load is waited for on every
     // path), then the class rows and the plane three ahead are fetched, in
     // that order.
-    for(int z = z0; z < z1; z += 2) {
+    // The loop body holds both halves with no exit between them (an odd last
+    // plane is handled after the loop), so every path into the loop head has
+    // issued the same loads in the same order and the compiler's wait counts
+    // there leave the other register set's loads in flight.
+    auto half_a = [&](int z) {
         put(ra, (z + 1) % 3);  // plane z+1
         if constexpr(CLS) crows[tid] = rrow;
         fetch_rows(z + 1, z + 1 < z1);
@@ -508,7 +512,10 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
         preload(pa, z + 2, z + 2 < z1);
         fa = face_pre(z + 2, z + 2 < z1);
         __syncthreads();
-        if(z + 1 >= z1) break;
+    };
+    int z = z0;
+    for(; z + 1 < z1; z += 2) {
+        half_a(z);
         put(rb, (z + 2) % 3);  // plane z+2
         if constexpr(CLS) crows[tid] = rrow;
         fetch_rows(z + 2, z + 2 < z1);
@@ -519,6 +526,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
         fb = face_pre(z + 3, z + 3 < z1);
         __syncthreads();
     }
+    if(z < z1) half_a(z);
 }
 
 // Call f(tile, z0, z1) for every tile and plane range of run r.
