@@ -5,10 +5,12 @@ plugin kind of the monodomain step through its dlopen ABI: the ionic model
 ([ode_solver]), ASSEMBLY_MATRIX ([assembly_matrix]), UPDATE_MONODOMAIN
 ([update_monodomain]) and SOLVE_LINEAR_SYSTEM ([linear_system_solver]).
 
-  - Assembly and update are exact: with one OpenMP thread (the reference's
-    assembly inserts elements in thread order) a run whose assembly and update
-    come from the plugins writes the same Vm files, bit for bit, as the
-    all-reference run (both with the reference's CPU CG and ionic model).
+  - Assembly and update: with one OpenMP thread (the reference's assembly
+    inserts elements in thread order) a run whose assembly and update come
+    from the plugins writes the same Vm files as the all-reference run (both
+    with the reference's CPU CG and ionic model): bit for bit for isotropic
+    rows, to the rounding of the CG's row sums for anisotropic rows, whose
+    elements are listed in another order.
   - All four kinds at config 1 (64x64x16 TT2006, 50 ms) against the
     reference's own run, to the tolerances of the FP64 tissue tests.
   - The GPU half of the ODE ABI (set_model_initial_conditions_gpu /
@@ -90,8 +92,9 @@ def test_assembly_and_update_plugins_are_bit_exact(case, tmp_path):
     whose matrix comes from mono_{homogeneous,anisotropic}_sigma_assembly_matrix
     (rows assembled on the device, written back as element lists in the
     reference's insertion order) and whose b comes from mono_update_monodomain
-    (alpha * Vm on the device) writes the same V_it_<k>.bin files, bit for bit,
-    as the all-reference run."""
+    (alpha * Vm on the device) writes the same V_it_<k>.bin files as the
+    all-reference run: bit for bit for the isotropic rows, within 1e-9 mV for
+    the anisotropic ones (see below)."""
     _need(REF_EXE, MONO)
     if case == "iso":
         shape, model, T, every, kw, asm = (16, 16, 4), "tt2006", 10.0, 50, {}, "homogeneous_sigma"
@@ -132,7 +135,8 @@ def test_assembly_and_update_plugins_are_bit_exact(case, tmp_path):
 def test_reference_core_with_all_plugin_kinds_config1(tmp_path):
     """Config 1 in full (64x64x16 TT2006, isotropic 0.13 S/m, 50 ms) in the
     reference's MonoAlg3D with the ionic model, assembly, update and CG all
-    from the plugins: Vm of the probe cells within 1e-5 mV and activation times
+    from the plugins: Vm of the probe cells within 1e-6 mV (1e-8 of the 100 mV
+    scale of Vm) and activation times
     within 1e-3 ms of the reference's own run; the drop-in rate is printed."""
     _need(REF_EXE, MONO, os.path.join(PLUG, "libmono_hip_tt2006.so"))
     announce("reference core + all four plugin kinds, config 1 64x64x16 tt2006 50 ms")
@@ -156,7 +160,7 @@ def test_reference_core_with_all_plugin_kinds_config1(tmp_path):
     print(f"[drop-in] config 1 through the reference core with all four plugin kinds: max |dV| = {dv:.3e} mV, "
           f"max |dLAT| = {dlat:.3e} ms, resolution time {res} s, "
           f"{(n * 2501 / res) if res else float('nan'):.3e} cell-steps/s", flush=True)
-    assert dv < 1e-5, dv
+    assert dv < 1e-6, dv
     assert dlat <= 1e-3, dlat
 
 

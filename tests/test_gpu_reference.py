@@ -3,10 +3,11 @@
 The fixtures were written by the reference binary built from its sources
 (tests/golden/make_golden.py): Vm of 12 probe cells at every saved step, Vm
 maps at fixed times and activation times (first upward crossing of -30 mV,
-linearly interpolated between saved samples).  The reference's CG stops on
-||r||^2 <= 1e-16, so FP64 runs agree to solver round-off: probe Vm within
-1e-5 mV, activation times within 1e-3 ms.  FP32 states + FP32 CG inside FP64
-defect correction must keep activation times within 0.5 ms of the reference.
+linearly interpolated between saved samples).  FP64 runs must agree with the
+reference's to 1e-8 relative: Vm within 1e-6 mV (1e-8 of the 100 mV scale of
+Vm; the measured differences are 1e-13 to 5e-9 mV, printed per fixture),
+activation times within 1e-3 ms.  FP32 states + FP32 CG inside FP64 defect
+correction must keep activation times within 0.5 ms of the reference.
 """
 import os
 import sys
@@ -23,6 +24,7 @@ sys.path.insert(0, GOLD)
 from make_golden import lat_from  # noqa: E402  (same activation-time rule as the fixtures)
 
 DT = 0.02
+FP64_VM_TOL = 1e-6  # mV: 1e-8 relative to the 100 mV scale of Vm
 
 
 def _run(name, model, precision, aniso=False):
@@ -77,13 +79,11 @@ def _check(g, snaps, st, vm_tol, lat_tol, vm_times):
 @pytest.mark.parametrize("name,times", [("tissue_cable_tt2006_300", (20, 60)), ("tissue_sheet_tt2006_48x32", (20, 40))])
 def test_cable_and_sheet_fp64_match_reference(name, times):
     """One-cell-thick domains: the reference's cable mesh (300 x 1 x 1, 60 ms)
-    and a 48 x 32 x 1 sheet (40 ms).  Over the 3,000 steps of the cable the
-    per-step solver differences (|r|/alpha <= 1.4e-7 mV) may add up to 4e-4 mV:
-    Vm within 1e-4 mV, activation times within 1e-3 ms."""
+    and a 48 x 32 x 1 sheet (40 ms), to the FP64 tolerance."""
     import monoalg3d_c_amd as m
     announce(f"reference {name} fp64")
     g, snaps, st = _run(name, "tt2006", m.FP64)
-    _check(g, snaps, st, 1e-4, 1e-3, times)
+    _check(g, snaps, st, FP64_VM_TOL, 1e-3, times)
 
 
 def test_config1_fp64_matches_reference():
@@ -92,7 +92,7 @@ def test_config1_fp64_matches_reference():
     announce("reference config1 64x64x16 tt2006 fp64 50 ms")
     g, snaps, st = _run("tissue_config1_tt2006_64x64x16", "tt2006", m.FP64)
     assert st["class_operator"] == 1 and st["stencil_points"] == 7
-    _check(g, snaps, st, 1e-5, 1e-3, (20,))
+    _check(g, snaps, st, FP64_VM_TOL, 1e-3, (20,))
 
 
 def test_config1_fp32_refine_lat_within_half_ms():
@@ -117,7 +117,7 @@ def test_tissue_fp64_matches_reference(name, model, aniso, vm_times):
     import monoalg3d_c_amd as m
     announce(f"reference {name} fp64")
     g, snaps, st = _run(name, model, m.FP64, aniso)
-    _check(g, snaps, st, 1e-5, 1e-3, vm_times)
+    _check(g, snaps, st, FP64_VM_TOL, 1e-3, vm_times)
 
 
 @pytest.mark.parametrize("name,model,aniso", [
@@ -182,7 +182,7 @@ def test_reference_core_with_hip_plugins(tmp_path):
     """The reference's own MonoAlg3D (built from its sources) with its ionic
     model and linear solver swapped for the plugins over libmonoalg_hip.so
     (monoalg3d_c_amd/plugins): the same run as the 32x32x8 TT2006 fixture,
-    Vm within 1e-5 mV and activation times within 1e-3 ms."""
+    Vm within 1e-6 mV and activation times within 1e-3 ms."""
     import subprocess
     from test_abi import PLUGINS, REF_EXE, _plugin_ini
     from make_golden import read_snapshots
@@ -198,7 +198,9 @@ def test_reference_core_with_hip_plugins(tmp_path):
     assert r.returncode == 0, r.stderr[-2000:]
     snaps = read_snapshots(str(tmp_path), shape, every, int(round(20.0 / DT)) + 1)
     probes = g["probe_cells"]
-    assert np.abs(snaps[:, probes] - g["probe_vm"]).max() < 1e-5
+    dv = np.abs(snaps[:, probes] - g["probe_vm"]).max()
+    print(f"[reference core + plugins] probe max |dV| = {dv:.3e} mV", flush=True)
+    assert dv < FP64_VM_TOL, dv
     lat = lat_from(snaps, every)
     both = np.isfinite(g["lat"])
     assert np.array_equal(np.isfinite(lat), both)
@@ -251,7 +253,7 @@ def test_reference_ini_drop_in(tmp_path):
     """The reference's own configuration for the fibrotic ORd case (custom mesh
     omitting 10% of the cells, fibre file, anisotropic assembly) loaded by
     monoalg3d_c_amd.inputs and run on the device, writing V_it_<k>.bin as the
-    reference does: the same snapshots as the reference's run within 1e-5 mV,
+    reference does: the same snapshots as the reference's run within 1e-6 mV,
     activation times within 1e-3 ms."""
     from make_golden import case_files, read_snapshots
     from monoalg3d_c_amd.inputs import load_ini, run
@@ -266,10 +268,12 @@ def test_reference_ini_drop_in(tmp_path):
     assert st["nan_detected"] == 0 and st["class_operator"] == 0
     snaps = read_snapshots(str(tmp_path), shape, every, len(g["probe_vm"]) * every - every + 1)
     live = act.astype(bool)
-    assert np.abs(snaps[:, g["probe_cells"]][:, live[g["probe_cells"]]] - g["probe_vm"][:, live[g["probe_cells"]]]).max() < 1e-5
+    dv = np.abs(snaps[:, g["probe_cells"]][:, live[g["probe_cells"]]] - g["probe_vm"][:, live[g["probe_cells"]]]).max()
+    print(f"[ini drop-in] probe max |dV| = {dv:.3e} mV", flush=True)
+    assert dv < FP64_VM_TOL, dv
     for tms in (5, 10):
         k = int(round(tms / (every * DT)))
-        assert np.abs(snaps[k][live] - g[f"vm_t{tms}"][live]).max() < 1e-5
+        assert np.abs(snaps[k][live] - g[f"vm_t{tms}"][live]).max() < FP64_VM_TOL
     lat = lat_from(snaps, every)
     both = live & np.isfinite(g["lat"])
     assert np.array_equal(np.isfinite(lat[live]), np.isfinite(g["lat"][live]))
