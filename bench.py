@@ -487,7 +487,7 @@ def make_line(args, shape, neq, res, res3, errors, cpu, dropin) -> dict:
     # phase, timed with HIP events on the library's stream.
     hk = "fp32" if mixed else "fp64"
     d = rf[hk]["ionic"]
-    kname = "k_ionic_step32<1>" if hk == "fp32" else "k_ionic_step64<1, true>"  # ORd 2011 = model 1
+    kname = "k_ionic_step32<1, false>" if hk == "fp32" else "k_ionic_step64<1, true, false>"  # ORd 2011 = model 1, no mask
     roofline = {"bound": "hbm", "achieved": round(d["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(d["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname, shape),
                 "kernel": kname, "launch_ms": round(res[hk]["ionic_ms"], 4), "bytes_per_launch": int(d["bytes"])}

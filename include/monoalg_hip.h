@@ -141,6 +141,8 @@ typedef struct mono_stats {
     int stencil_points;             /* stored non-zeros per row */
     double ionic_ms_total;          /* device time in the ionic kernel (timing on) */
     double solve_ms_total;          /* device time in the diffusion solve (timing on) */
+    int ionic_block;                /* threads per block of the ionic step kernel picked by the
+                                       launch-shape tuner (0: not tuned yet, or tuning off) */
 } mono_stats;
 
 typedef struct mono_tissue mono_tissue;

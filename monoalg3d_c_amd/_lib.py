@@ -62,7 +62,7 @@ class Stats(C.Structure):
         ("last_cg_iterations", C.c_int), ("last_cg_error", C.c_double), ("last_outer", C.c_int),
         ("inner_iterations_total", C.c_int64), ("outer_sweeps_total", C.c_int64), ("nan_detected", C.c_int),
         ("class_operator", C.c_int), ("stencil_points", C.c_int), ("ionic_ms_total", C.c_double),
-        ("solve_ms_total", C.c_double),
+        ("solve_ms_total", C.c_double), ("ionic_block", C.c_int),
     ]
 
     def as_dict(self) -> dict:

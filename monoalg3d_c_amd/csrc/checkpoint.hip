@@ -57,7 +57,7 @@ using namespace mono;
 namespace {
 
 constexpr char kMagic[8] = {'M', 'O', 'N', 'O', 'H', 'I', 'P', 'C'};
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;  // 2: mono_stats gained ionic_block
 constexpr size_t kSolverBytes = 88, kTimeInfoBytes = 32;  // struct monodomain_solver, struct time_info
 
 struct File {

@@ -30,6 +30,7 @@ template <typename T> struct IonicLaunch {
     T dt;                   // ODE step
     int num_steps;          // ODE steps per PDE step
     int first_step;         // read V from sv row 0 instead of v (t = 0: no Vm mirror)
+    int block;              // step kernels: threads per block (0: the built-in choice)
     int write_vm_row;       // also store V* in sv row 0
     // adaptive Euler (launch_ionic_adaptive): per-cell dt, time_new and
     // previous_dt in three FP64 rows of `pitch`; each cell advances from its

@@ -226,6 +226,17 @@ int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out) {
             mono_tissue_destroy(t);
             return -1;
         }
+    {
+        const char *e = getenv("MONO_TUNE");
+        if(!desc->ode_adaptive && !(e && atoi(e) == 0)) {
+            if(hipEventCreate(&t->tune.e0) != hipSuccess || hipEventCreate(&t->tune.e1) != hipSuccess) {
+                mono_set_error("hipEventCreate failed");
+                mono_tissue_destroy(t);
+                return -1;
+            }
+            t->tune.trial = 0;
+        }
+    }
     auto fail = [&](void) {
         mono_tissue_destroy(t);
         return -1;
@@ -341,6 +352,8 @@ int mono_tissue_destroy(mono_tissue *t) {
         if(p && hipFree(p) != hipSuccess) rc = -1;
     for(int k = 0; k < 3; k++)
         if(t->ev[k] && hipEventDestroy(t->ev[k]) != hipSuccess) rc = -1;
+    for(hipEvent_t e : {t->tune.e0, t->tune.e1})
+        if(e && hipEventDestroy(e) != hipSuccess) rc = -1;
     if(t->stream && hipStreamDestroy(t->stream) != hipSuccess) rc = -1;
     delete t;
     return rc;
@@ -424,7 +437,7 @@ int mono_tissue_graph_counts(const mono_tissue *t, int64_t *launches, int64_t *c
     return 0;
 }
 
-static int launch_ionic(mono_tissue *t, double stim_amp) {
+static int launch_ionic(mono_tissue *t, double stim_amp, int block) {
     const mono_tissue_desc &d = t->d;
     if(d.precision == MONO_FP64) {
         IonicLaunch<double> a{};
@@ -447,6 +460,7 @@ static int launch_ionic(mono_tissue *t, double stim_amp) {
         a.dt = t->dt_ode;
         a.num_steps = t->num_ode_steps;
         a.first_step = t->st.steps == 0;
+        a.block = block;
         if(d.ode_adaptive) {
             set_adaptive(t, a);
             MONO_CHECK(launch_ionic_adaptive(d.model, a, t->stream));
@@ -474,6 +488,7 @@ static int launch_ionic(mono_tissue *t, double stim_amp) {
         a.dt = (float)t->dt_ode;
         a.num_steps = t->num_ode_steps;
         a.first_step = t->st.steps == 0;
+        a.block = block;
         if(d.ode_adaptive) {
             set_adaptive(t, a);
             MONO_CHECK(launch_ionic_adaptive(d.model, a, t->stream));
@@ -482,6 +497,38 @@ static int launch_ionic(mono_tissue *t, double stim_amp) {
         }
     }
     return 0;
+}
+
+// The block size for this step's ionic launch from the tuner: the previous
+// trial's time is read back (its launch finished within the previous step's
+// solve, which waited on the device), then the next candidate, or the fastest
+// once every candidate has run twice after the first step.
+static int tuned_block(mono_tissue *t, bool &timed) {
+    auto &u = t->tune;
+    timed = false;
+    if(u.trial < 0) return u.chosen;
+    if(u.pending >= 0) {
+        float ms = 0.f;
+        MONO_CHECK(hipEventSynchronize(u.e1));
+        MONO_CHECK(hipEventElapsedTime(&ms, u.e0, u.e1));
+        const int c = u.pending;
+        u.best_ms[c] = (u.best_ms[c] == 0.f || ms < u.best_ms[c]) ? ms : u.best_ms[c];
+        u.pending = -1;
+    }
+    if(t->st.steps == 0) return u.chosen;  // the first step reads V from the state rows: not timed
+    if(u.trial >= 6) {
+        int best = 0;
+        for(int c = 1; c < 3; c++)
+            if(u.best_ms[c] < u.best_ms[best]) best = c;
+        u.chosen = u.cand[best];
+        u.trial = -1;
+        t->st.ionic_block = u.chosen;
+        return u.chosen;
+    }
+    u.pending = u.trial % 3;
+    u.trial++;
+    timed = true;
+    return u.cand[u.pending];
 }
 
 int mono_tissue_step(mono_tissue *t, int nsteps) {
@@ -507,9 +554,14 @@ int mono_tissue_step(mono_tissue *t, int nsteps) {
             t->stim_start = strtod(buf, nullptr);
         }
 
+        bool tuned = false;
+        const int block = tuned_block(t, tuned);
+        if(block < 0) return -1;
+        if(tuned) MONO_CHECK(hipEventRecord(t->tune.e0, t->stream));
         if(t->timing) MONO_CHECK(hipEventRecord(t->ev[0], t->stream));
-        if(launch_ionic(t, amp)) return -1;
+        if(launch_ionic(t, amp, block)) return -1;
         if(t->timing) MONO_CHECK(hipEventRecord(t->ev[1], t->stream));
+        if(tuned) MONO_CHECK(hipEventRecord(t->tune.e1, t->stream));
         CgStats cs{};
         int rc;
         if(d.precision == MONO_FP64) rc = t->cg.solve_fp64(t->op.view64(), t->b, t->v, t->cgcfg, &cs, t->stream);

@@ -32,6 +32,18 @@ struct mono_tissue {
     mono::ApdTracker apd;
     hipStream_t stream = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // Launch-shape tuner of the ionic step kernel: the first steps of the run
+    // time each candidate block size twice (per-cell arithmetic does not depend
+    // on it, so the run's results are the same whichever is picked), then the
+    // fastest is kept.  Off for the adaptive solver and with MONO_TUNE=0.
+    struct IonicTuner {
+        int cand[3] = {64, 128, 256};
+        float best_ms[3] = {0.f, 0.f, 0.f};
+        int trial = -1;    // next trial (-1: tuning off or done)
+        int pending = -1;  // candidate whose launch the events bracket
+        int chosen = 0;    // block size in use (0: the built-in choice)
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+    } tune;
     int timing = 0;
     int num_ode_steps = 1;
     double dt_ode = 0.02;
