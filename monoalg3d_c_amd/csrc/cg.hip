@@ -20,6 +20,8 @@
 // r = b - A x is formed in FP64 with the FP64 operator; an FP32 CG solves
 // A d = r to a relative tolerance with FP64 dot accumulation; x += d in FP64.
 #include <stdlib.h>
+#include <string.h>
+
 #include <type_traits>
 #include <vector>
 
@@ -256,6 +258,9 @@ static_assert(kBlock % kTX == 0 && (kTX * kTY) % kBlock == 0 && 2 * kTY <= kWave
 constexpr int kNLD = (kPL + kBlock - 1) / kBlock;
 constexpr int kPLS = kNLD * kBlock;  // LDS plane stride: every lane of every load set has a slot
 constexpr int kRPT = kTX * kTY / kBlock;  // rows per thread
+#ifndef MONO_PERCELL_PREFETCH
+#define MONO_PERCELL_PREFETCH 1
+#endif
 
 // Work split of the plane-loop kernels.  The items are (xy tile, z plane)
 // pairs in tile-major order (a tile's planes are consecutive); run r is the
@@ -414,6 +419,26 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
     auto face_pre = [&](int z, bool need) -> Pre {
         return epi.load((need && face_lane) ? fx + (int64_t)op.nx * (y0 + (tid % kTY)) + sz * z : 0);
     };
+    // per-cell rows: the rows of plane z are loaded at the end of the previous
+    // plane's compute, so their loads are in flight across the LDS staging and
+    // both barriers instead of being waited for row by row inside compute
+    // (read once per sweep: non-temporal loads).
+    // Slot arrays start at uniform addresses; a 32-bit cell offset lets every
+    // load use a scalar base (callers keep n <= 2^31).
+    constexpr bool PF = CM == 0 && MONO_PERCELL_PREFETCH;
+    TC crg[PF ? kRPT : 1][PF ? PAT : 1];
+    auto fetch_coef = [&](int z, bool need) {
+        if constexpr(PF) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for(int q = 0; q < kRPT; q++) {
+                const uint32_t cu = (uint32_t)(need ? cell_of(q, z) : 0);
+#pragma unroll
+                for(int k = 0; k < PAT; k++) crg[q][k] = __builtin_nontemporal_load(op.percell + (int64_t)k * op.n + cu);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
     // plane z from LDS (planes z-1, z, z+1 and, when needed, its class rows staged)
     auto compute = [&](int z, const Pre(&pre)[kRPT], Pre fpre) {
         const TV *const P3[3] = {pl + ((z + 2) % 3) * kPLS, pl + (z % 3) * kPLS, pl + ((z + 1) % 3) * kPLS};
@@ -427,8 +452,10 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
                 for(int k = 0; k < PAT; k++) crow[k] = row[k];
             }
         }
-        // per-cell rows: one row at a time (its PAT coefficient loads are in flight)
-#pragma unroll CLS ? kRPT : 1
+        // per-cell rows without prefetch: one row at a time (its PAT
+        // coefficient loads are in flight)
+        constexpr int kUq = CLS || PF ? kRPT : 1;
+#pragma unroll kUq
         for(int q = 0; q < kRPT; q++) {
             const int rr = ty + q * (kBlock / kTX);
             const int gy = y0 + rr;
@@ -437,9 +464,14 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
             const int base = (rr + 1) * kW + tx + 1;
             TV acc;
             double dg = 0.0;  // the row's diagonal (Jacobi epilogues only)
-            if constexpr(CM == 0) {
-                // slot arrays start at uniform addresses; a 32-bit cell offset
-                // lets every load use a scalar base (callers keep n <= 2^31)
+            if constexpr(PF) {
+                acc = TV(0);
+#pragma unroll
+                for(int k = 0; k < PAT; k++)
+                    acc += (TV)crg[q][k] *
+                           P3[RowPattern<PAT>::o[k][2] + 1][base + RowPattern<PAT>::o[k][1] * kW + RowPattern<PAT>::o[k][0]];
+                if constexpr(Epi::kDiag) dg = (double)crg[q][0];
+            } else if constexpr(CM == 0) {
                 const uint32_t cu = (uint32_t)c;
                 acc = TV(0);
 #pragma unroll
@@ -494,6 +526,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
     Pre fa = face_pre(z0, true);
     preload(pb, z0 + 1, z0 + 1 < z1);
     Pre fb = face_pre(z0 + 1, z0 + 1 < z1);
+    fetch_coef(z0, true);
     // Each half: the plane and class rows fetched earlier go into LDS (the
     // class rows unconditionally, so that their load is waited for on every
     // path), then the class rows and the plane three ahead are fetched, in
@@ -511,6 +544,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
         compute(z, pa, fa);
         preload(pa, z + 2, z + 2 < z1);
         fa = face_pre(z + 2, z + 2 < z1);
+        fetch_coef(z + 1, z + 1 < z1);
         __syncthreads();
     };
     int z = z0;
@@ -524,6 +558,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
         compute(z + 1, pb, fb);
         preload(pb, z + 3, z + 3 < z1);
         fb = face_pre(z + 3, z + 3 < z1);
+        fetch_coef(z + 2, z + 2 < z1);
         __syncthreads();
     }
     if(z < z1) half_a(z);
@@ -1054,6 +1089,7 @@ template <typename TC, class F> static void with_op(const OpView<TC> &op, F &&f)
 // most kMaxBlocks, the partial-sum slots; at least kMinRun items each), in
 // whole rounds of the XCD dispatch order.
 constexpr int kMinRun = 8;
+constexpr int kFill = 2;  // planes of pipeline fill at the start of a segment
 template <typename TC> static TileGeom make_geom(const OpView<TC> &op, int slots) {
     TileGeom g{};
     const int L[3] = {op.nx, op.ny, op.nz};
@@ -1064,34 +1100,50 @@ template <typename TC> static TileGeom make_geom(const OpView<TC> &op, int slots
     g.ntx = (op.nx + kTX - 1) / kTX;
     g.nty = (op.ny + kTY - 1) / kTY;
     g.ntiles = g.ntx * g.nty;
-#if MONO_SLAB_RUNS
-    // slab mode: every tile of the plane in each of nslab z slabs, one run per
-    // block, all runs resident at once; the runs of a slab are consecutive,
-    // so an XCD's blocks march neighbouring tiles through the same planes
-    // together and their shared halo lines are fetched from HBM once
-    if(g.ntiles <= slots) {
-        int64_t nslab = slots / g.ntiles;
-        if(nslab > op.nz) nslab = op.nz;
-        g.L = (int)((op.nz + nslab - 1) / nslab);
-        nslab = (op.nz + g.L - 1) / g.L;
-        g.nruns = (int)(g.ntiles * nslab);
-        if(g.nruns <= kMaxBlocks) {
-            g.per = 0;
-            int64_t nb = g.nruns;
-            if(nb >= 16) nb = (nb + 7) / 8 * 8;
-            g.nblocks = (int)nb;
-            return g;
-        }
-        g.L = 0;
-    }
-#endif
+    // item ranges: one contiguous run of (tile, plane) items per block
     const int64_t nitems = (int64_t)g.ntiles * op.nz;
     int64_t nb = (nitems + kMinRun - 1) / kMinRun;
     if(nb > slots) nb = slots;
     if(nb > kMaxBlocks) nb = kMaxBlocks;
     if(nb >= 16) nb = nb / 8 * 8;
     if(nb < 1) nb = 1;
-    g.per = (int)((nitems + nb - 1) / nb);
+    const int per = (int)((nitems + nb - 1) / nb);
+#if MONO_SLAB_RUNS
+    // slab mode: every tile of the plane in each of nslab z slabs, one run per
+    // block, all runs resident at once; the runs of a slab are consecutive,
+    // so an XCD's blocks march neighbouring tiles through the same planes
+    // together and their shared halo lines are fetched from HBM once.  Its
+    // runs are whole slabs, so when the resident slots are not close to a
+    // multiple of the tile count its longest run exceeds the item-range one:
+    // the split with the shorter longest run wins, a run of item ranges
+    // counted with two pipeline fills (it may span two tiles) and a quarter
+    // more for the halo lines its unaligned neighbours fetch again (measured:
+    // 16M-cell box, FP32 class sweep 74.5 us as 1050 slab runs of 34 planes
+    // against 80.5 us as item runs of 28; FP64 class residual 163 us as 700
+    // slab runs of 50 planes on 1024 slots against 130 us as item runs of 35).
+    if(g.ntiles <= slots) {
+        int64_t nslab = slots / g.ntiles;
+        if(nslab > op.nz) nslab = op.nz;
+        const int Ls = (int)((op.nz + nslab - 1) / nslab);
+        nslab = (op.nz + Ls - 1) / Ls;
+        const double cost_slab = Ls + kFill, cost_item = 1.25 * (per + 2 * kFill);
+        const char *force = getenv("MONO_SPLIT");  // "slab" / "item": measurement override
+        const bool slab = force && !strcmp(force, "slab") ? true
+                          : force && !strcmp(force, "item") ? false
+                                                             : cost_slab <= cost_item;
+        if(slab && g.ntiles * nslab <= kMaxBlocks) {
+            g.L = Ls;
+            g.nruns = (int)(g.ntiles * nslab);
+            g.per = 0;
+            int64_t nbs = g.nruns;
+            if(nbs >= 16) nbs = (nbs + 7) / 8 * 8;
+            g.nblocks = (int)nbs;
+            return g;
+        }
+    }
+#endif
+    g.L = 0;
+    g.per = per;
     g.nruns = (int)((nitems + g.per - 1) / g.per);
     g.nblocks = (int)nb;
     return g;
@@ -1268,6 +1320,13 @@ static int launch_stencil(const OpView<TC> &op, const TV *x, const TV *rz, TV *p
                     C, P, (int)FUSE, Epi::NR, slots[dev], per_cu, cus);
     }
     const TileGeom g = make_geom(op, slots[dev]);
+    if(getenv("MONO_TRACE_SLOTS")) {
+        static bool said[64] = {};
+        if(!said[dev])
+            fprintf(stderr, "[mono]   %dx%dx%d: %s, %d runs of %d %s\n", op.nx, op.ny, op.nz, g.L ? "slab runs" : "item runs", g.nruns,
+                    g.L ? g.L : g.per, g.L ? "planes" : "items");
+        said[dev] = true;
+    }
     K<<<g.nblocks, kBlock, 0, s>>>(op, g, x, rz, p_out, epi, cs, mode, k, Pin, nprev, Pout, sc, so);
     return g.nruns;
 }
