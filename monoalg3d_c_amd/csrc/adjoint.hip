@@ -71,6 +71,27 @@ __global__ __launch_bounds__(256) void k_copy_vh(const double *v, const T *h, do
 //   dL/dg_c -= sum over the six faces of c: dc/dg_c (w_c - w_j)(V_{n+1,c} - V_{n+1,j})
 //   ionic adjoint of the forward-Euler Mitchell-Schaeffer step at (V_n, h_n):
 //   lambda_n = alpha w dV*/dV, mu_n = alpha w dV*/dh + mu dh'/dh, dL/ds += mu dh'/ds
+// two device-to-device copies in one launch (8-byte words): checkpoint save
+// and restore of v and the state rows.  A kernel rather than hipMemcpyAsync:
+// at 65k cells the runtime's device copies cost far more than the bytes moved.
+__global__ __launch_bounds__(256) void k_copy2(const uint64_t *s0, uint64_t *d0, int64_t n0, const uint64_t *s1, uint64_t *d1,
+                                               int64_t n1) {
+    for(int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n0 + n1; i += (int64_t)gridDim.x * 256) {
+        if(i < n0) d0[i] = s0[i];
+        else d1[i - n0] = s1[i - n0];
+    }
+}
+
+int copy2(const void *s0, void *d0, size_t b0, const void *s1, void *d1, size_t b1, hipStream_t s) {
+    const int64_t n0 = (int64_t)(b0 / 8), n1 = (int64_t)(b1 / 8);
+    int64_t g = (n0 + n1 + 255) / 256;
+    if(g > 4096) g = 4096;
+    if(g < 1) g = 1;
+    k_copy2<<<(unsigned)g, 256, 0, s>>>((const uint64_t *)s0, (uint64_t *)d0, n0, (const uint64_t *)s1, (uint64_t *)d1, n1);
+    MONO_CHECK(hipGetLastError());
+    return 0;
+}
+
 __global__ __launch_bounds__(256) void k_adjoint_step(int nx, int ny, int nz, const uint8_t *active, const double *g, double sx,
                                                       double sy, double sz, double mx, double my, double mz, const double *w,
                                                       const double *wprev, double *wnext, const double *V1, const double *V,
@@ -172,8 +193,12 @@ int check_spec(const mono_tissue_desc *d, int nsteps, const int64_t *sensors, in
     return 0;
 }
 
+// v and the state rows at a checkpoint: views into one allocation for all
+// checkpoints of a run (one hipMalloc instead of two per checkpoint; each
+// allocation costs about a millisecond, as much as several steps at 65k cells)
 struct Checkpoint {
-    DevBuf v, sv;
+    double *v = nullptr;
+    void *sv = nullptr;
     double cur_time = 0.0, stim_start = 0.0;
     int64_t steps = 0;
     // the mixed solve's sweep schedule and polling sizes: a recomputed segment
@@ -183,12 +208,7 @@ struct Checkpoint {
 };
 
 int save(mono_tissue *t, Checkpoint &c, size_t sv_bytes) {
-    if(!c.v.p) {
-        MONO_CHECK(hipMalloc(&c.v.p, sizeof(double) * t->n));
-        MONO_CHECK(hipMalloc(&c.sv.p, sv_bytes));
-    }
-    MONO_CHECK(hipMemcpyAsync(c.v.p, t->v, sizeof(double) * t->n, hipMemcpyDeviceToDevice, t->stream));
-    MONO_CHECK(hipMemcpyAsync(c.sv.p, t->sv, sv_bytes, hipMemcpyDeviceToDevice, t->stream));
+    if(copy2(t->v, c.v, sizeof(double) * t->n, t->sv, c.sv, sv_bytes, t->stream)) return -1;
     c.cur_time = t->cur_time;
     c.stim_start = t->stim_start;
     c.steps = t->st.steps;
@@ -199,8 +219,7 @@ int save(mono_tissue *t, Checkpoint &c, size_t sv_bytes) {
 }
 
 int restore(mono_tissue *t, const Checkpoint &c, size_t sv_bytes) {
-    MONO_CHECK(hipMemcpyAsync(t->v, c.v.p, sizeof(double) * t->n, hipMemcpyDeviceToDevice, t->stream));
-    MONO_CHECK(hipMemcpyAsync(t->sv, c.sv.p, sv_bytes, hipMemcpyDeviceToDevice, t->stream));
+    if(copy2(c.v, t->v, sizeof(double) * t->n, c.sv, t->sv, sv_bytes, t->stream)) return -1;
     t->cur_time = c.cur_time;
     t->st.time = c.cur_time;
     t->stim_start = c.stim_start;
@@ -266,6 +285,18 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
     const int nseg = nsteps > 0 ? (nsteps + K - 1) / K : 0;
     std::vector<Checkpoint> ck(store_all ? 0 : nseg);
     const size_t nb = sizeof(double) * n;
+    DevBuf ckpool;
+    if(!ck.empty()) {
+        const size_t vb = (nb + 255) / 256 * 256, per = vb + (sv_bytes + 255) / 256 * 256;
+        if(hipMalloc(&ckpool.p, per * ck.size()) != hipSuccess) {
+            mono_set_error("mono_sensor_gradient: no device memory for %zu checkpoints (%.2f GB)", ck.size(), per * ck.size() / 1e9);
+            return -1;
+        }
+        for(size_t i = 0; i < ck.size(); i++) {
+            ck[i].v = (double *)(ckpool.as<char>() + i * per);
+            ck[i].sv = ckpool.as<char>() + i * per + vb;
+        }
+    }
     DevBuf Vseg, Hseg;  // states of one segment (or of the whole run)
     if(want_grad) {
         if(hipMalloc(&Vseg.p, nb * (K + 1)) != hipSuccess || hipMalloc(&Hseg.p, nb * (K + 1)) != hipSuccess) {
@@ -275,6 +306,12 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
         }
     }
     if(want_grad && store_all && copy_vh(t, Vseg.as<double>(), Hseg.as<double>())) return -1;
+    const bool trace = getenv("MONO_TRACE_ADJOINT") != nullptr;
+    auto now = [&]() {
+        if(trace) (void)hipStreamSynchronize(t->stream);
+        return std::chrono::steady_clock::now();
+    };
+    const auto f0 = now();
     for(int k = 0; k < nsteps; k++) {
         if(!store_all && k % K == 0 && save(t, ck[k / K], sv_bytes)) return -1;
         if(k % record_every == 0 && record(t, idx, nsens, tmp, &tr[(size_t)(k / record_every) * nsens])) return -1;
@@ -288,6 +325,7 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
             return -1;
     }
     if(nsteps % record_every == 0 && record(t, idx, nsens, tmp, &tr[(size_t)(nsteps / record_every) * nsens])) return -1;
+    const double t_fw = std::chrono::duration<double>(now() - f0).count();
     if(traces) memcpy(traces, tr.data(), sizeof(double) * tr.size());
     if(!target) return 0;
 
@@ -332,13 +370,8 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
     const OpView<double> op = t->op.view64();
     bool live = false;  // lambda, mu non-zero yet
     // MONO_TRACE_ADJOINT=1: phase times and adjoint CG iterations on stderr
-    const bool trace = getenv("MONO_TRACE_ADJOINT") != nullptr;
     double t_re = 0.0, t_adj = 0.0;
     long adj_its = 0, adj_solves = 0;
-    auto now = [&]() {
-        if(trace) (void)hipStreamSynchronize(t->stream);
-        return std::chrono::steady_clock::now();
-    };
     for(int seg = nseg - 1; seg >= 0; seg--) {
         const int k0 = seg * K, k1 = (k0 + K < nsteps) ? k0 + K : nsteps;
         auto c0 = now();
@@ -382,8 +415,8 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
     }
     MONO_CHECK(hipStreamSynchronize(t->stream));
     if(trace)
-        fprintf(stderr, "[mono] adjoint: recompute %.3f s, adjoint sweep %.3f s, %ld solves, %.2f CG iterations per solve\n",
-                t_re, t_adj, adj_solves, adj_solves ? (double)adj_its / adj_solves : 0.0);
+        fprintf(stderr, "[mono] adjoint: forward %.3f s, recompute %.3f s, adjoint sweep %.3f s, %ld solves, %.2f CG iterations per solve\n",
+                t_fw, t_re, t_adj, adj_solves, adj_solves ? (double)adj_its / adj_solves : 0.0);
     if(grad_g) MONO_CHECK(hipMemcpy(grad_g, gg.p, nb, hipMemcpyDeviceToHost));
     if(grad_scale) MONO_CHECK(hipMemcpy(grad_scale, gs.p, nb, hipMemcpyDeviceToHost));
     return 0;
