@@ -25,6 +25,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "aniso_row.h"
 #include "cg.h"
 
 namespace mono {
@@ -318,9 +319,11 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
                                              const TV *__restrict__ rz, TV beta, TV *__restrict__ p_out, const Epi &epi,
                                              double (&v)[Epi::NR]) {
     static_assert(9 * 27 <= kBlock, "one class-row coefficient per thread");
-    // CM: 0 per-cell rows, 1 class table, 2 code table (7-point rows)
+    // CM: 0 per-cell rows, 1 class table, 2 code table (7-point rows), 3 rows
+    // formed from neighbourhood masks (27-point)
     constexpr bool CLS = CM == 1;
     static_assert(CM != 2 || PAT == 7, "code tables hold 7-point rows");
+    static_assert(CM != 3 || PAT == 27, "mask rows are 27-point rows");
     __shared__ TV pl[3 * kPLS];
     __shared__ TC crows[CLS ? kBlock : 1];  // 9 x 27 class rows, padded to one slot per thread
     __shared__ TC ctab[CM == 2 ? kCodes * PAT : 1];
@@ -427,7 +430,16 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
     // load use a scalar base (callers keep n <= 2^31).
     constexpr bool PF = CM == 0 && MONO_PERCELL_PREFETCH;
     TC crg[PF ? kRPT : 1][PF ? PAT : 1];
+    // mask rows: the masks of plane z, loaded likewise a plane ahead
+    constexpr bool MM = CM == 3;
+    uint32_t mk[MM ? kRPT : 1];
     auto fetch_coef = [&](int z, bool need) {
+        if constexpr(MM) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for(int q = 0; q < kRPT; q++) mk[q] = __builtin_nontemporal_load(op.nbm + (uint32_t)(need ? cell_of(q, z) : 0));
+            __builtin_amdgcn_sched_barrier(0);
+        }
         if constexpr(PF) {
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -443,6 +455,18 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
     auto compute = [&](int z, const Pre(&pre)[kRPT], Pre fpre) {
         const TV *const P3[3] = {pl + ((z + 2) % 3) * kPLS, pl + (z % 3) * kPLS, pl + ((z + 1) % 3) * kPLS};
         TC crow[PAT];
+        // mask rows: the tensors of layers z-1, z, z+1 (clamped at the box
+        // faces, where no neighbour uses them), wave-uniform
+        TC tz[MM ? 3 : 1][6];
+        if constexpr(MM) {
+            const scalar_ptr<TC> lt = as_scalar(op.lt);
+#pragma unroll
+            for(int l = 0; l < 3; l++) {
+                const int zl = z - 1 + l < 0 ? 0 : (z - 1 + l >= op.nz ? op.nz - 1 : z - 1 + l);
+#pragma unroll
+                for(int k = 0; k < 6; k++) tz[l][k] = lt[zl * 6 + k];
+            }
+        }
         bool zf = true;
         if constexpr(CLS) {
             zf = z >= g.lo[2] && z <= g.hi[2];
@@ -454,7 +478,7 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
         }
         // per-cell rows without prefetch: one row at a time (its PAT
         // coefficient loads are in flight)
-        constexpr int kUq = CLS || PF ? kRPT : 1;
+        constexpr int kUq = CLS || PF || MM ? kRPT : 1;
 #pragma unroll kUq
         for(int q = 0; q < kRPT; q++) {
             const int rr = ty + q * (kBlock / kTX);
@@ -464,7 +488,16 @@ __device__ __forceinline__ void tile_segment(const OpView<TC> &op, const TileGeo
             const int base = (rr + 1) * kW + tx + 1;
             TV acc;
             double dg = 0.0;  // the row's diagonal (Jacobi epilogues only)
-            if constexpr(PF) {
+            if constexpr(MM) {
+                TC row[27];
+                mask_aniso_row<TC, true>(mk[q], tz, op.geo[0], op.geo[1], op.geo[2], op.geo[3], row);
+                acc = TV(0);
+#pragma unroll
+                for(int k = 0; k < PAT; k++)
+                    acc += (TV)row[slot_of(RowPattern<PAT>::o[k][0], RowPattern<PAT>::o[k][1], RowPattern<PAT>::o[k][2])] *
+                           P3[RowPattern<PAT>::o[k][2] + 1][base + RowPattern<PAT>::o[k][1] * kW + RowPattern<PAT>::o[k][0]];
+                if constexpr(Epi::kDiag) dg = (double)row[13];
+            } else if constexpr(PF) {
                 acc = TV(0);
 #pragma unroll
                 for(int k = 0; k < PAT; k++)
@@ -1074,12 +1107,15 @@ template <class F> static void with_pat(int P, F &&f) {
 
 // Call f(IC<S>, IC<PAT>) for the operator's class-row slot count and row pattern.
 // Call f(IC<CM>, IC<PAT>) for the operator's storage mode (0 per-cell rows,
-// 1 class table, 2 code table) and row pattern.
+// 1 class table, 2 code table, 3 mask rows: FP32 views only) and row pattern.
 template <typename TC, class F> static void with_op(const OpView<TC> &op, F &&f) {
     with_pat(op.pat, [&](auto P_) {
         if(op.table) f(IC<1>{}, P_);
         else if constexpr(decltype(P_)::value == 7) {
             if(op.codes) f(IC<2>{}, P_);
+            else f(IC<0>{}, P_);
+        } else if constexpr(decltype(P_)::value == 27 && std::is_same<TC, float>::value) {
+            if(op.nbm) f(IC<3>{}, P_);
             else f(IC<0>{}, P_);
         } else f(IC<0>{}, P_);
     });

@@ -76,8 +76,22 @@ template <typename T> struct OpView {
     const uint8_t *codes;       // code mode: [cell] row code (7-point rows only), else null
     const T *ctab;              // code mode: [code * pat + k], kCodes rows, else null
     const uint8_t *active;      // optional
+    // mask mode (27-point anisotropic rows formed in the kernel, see
+    // aniso_row.h): [cell] neighbourhood mask, [z][6] layer tensors, and
+    // dx, dy, dz, alpha; else null
+    const uint32_t *nbm;
+    const T *lt;
+    T geo[4];
 };
 
+// Mask mode: for 27-point anisotropic rows on per-layer (or uniform) tensors
+// without per-cell conductivity factors, a row is fixed by its layer and which
+// of its 26 neighbours exist and are active.  The FP32 view then carries a
+// 4-byte mask per cell and the inner solver's kernels form each row in FP32
+// registers (aniso_row.h) instead of reading 27 stored coefficients; enabled
+// only after the FP64 form of the same function reproduced every assembled
+// row bit for bit.  MONO_MASK_ROWS=0 keeps the stored FP32 rows.
+//
 // Code mode: when every 7-point row is fixed by which of its six neighbours
 // exist and are active (a uniform isotropic tensor on a box with inactive
 // cells, e.g. many instances packed with gaps, or holes), the row of a cell is
@@ -94,6 +108,10 @@ struct Operator {
     int pat = 27;          // pattern of the interior rows (class mode) or of all rows (per-cell)
     bool class_mode = false;
     bool code_mode = false;
+    bool mask32 = false;   // FP32 view in mask mode (the FP64 view keeps per-cell rows)
+    uint32_t *nbm = nullptr;
+    float *lt32 = nullptr;
+    float geo32[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     uint8_t *codes = nullptr;
     double *ctab64 = nullptr;
     float *ctab32 = nullptr;

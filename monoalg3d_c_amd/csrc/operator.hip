@@ -13,17 +13,22 @@
 // direction order.  Neighbour directions map to offsets FRONT/BACK = +z/-z,
 // TOP/DOWN = +y/-y, RIGHT/LEFT = +x/-x (cell.h:21-49).
 // A neighbour exists when it lies in the box and is active.
+// coefficients are formed exactly as the reference forms them: no FMA
+// contraction, also in the mask-row function of aniso_row.h, whose FP64 form
+// is checked here bit for bit against the assembled rows (the pragma precedes
+// the includes so that it covers the header's function bodies)
+#pragma clang fp contract(off)
+
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
 #include <vector>
 
+#include "aniso_row.h"
 #include "mono_common.h"
 #include "operator.h"
-
-// coefficients are formed exactly as the reference forms them: no FMA contraction
-#pragma clang fp contract(off)
 
 namespace mono {
 
@@ -367,7 +372,7 @@ __global__ __launch_bounds__(256) void k_compact(const double *coef27, int64_t n
     for(int k = 0; k < S; k++) {
         const double v = slots[k] >= 0 ? coef27[(int64_t)slots[k] * n + c] : 0.0;
         out64[(int64_t)k * n + c] = v;
-        out32[(int64_t)k * n + c] = (float)v;
+        if(out32) out32[(int64_t)k * n + c] = (float)v;
     }
 }
 
@@ -512,6 +517,102 @@ static int try_code_mode(const AssemblyParams &p, const double *coef27, const st
     return 1;
 }
 
+// Neighbourhood mask of each cell (see aniso_row.h): bit slot_of(o) when the
+// neighbour at offset o lies in the box and is active, bit 13 the cell's own
+// activity; 0 for an inactive cell.
+__global__ __launch_bounds__(256) void k_nb_mask(int nx, int ny, int nz, const uint8_t *active, uint32_t *nbm) {
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(c >= n) return;
+    const int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / ((int64_t)nx * ny));
+    uint32_t m = 0;
+    if(!active || active[c]) {
+        for(int k = 0; k < 27; k++) {
+            const int xx = x + k % 3 - 1, yy = y + (k / 3) % 3 - 1, zz = z + k / 9 - 1;
+            if(xx < 0 || xx >= nx || yy < 0 || yy >= ny || zz < 0 || zz >= nz) continue;
+            if(active && !active[xx + (int64_t)nx * (yy + (int64_t)ny * zz)]) continue;
+            m |= 1u << k;
+        }
+    }
+    nbm[c] = m;
+}
+
+// Rows re-formed from the masks in FP64 against the assembled rows, bit for bit.
+// lstride: 6 for per-layer tensors, 0 for a uniform one.
+__global__ __launch_bounds__(256) void k_verify_mask_rows(const double *coef27, int64_t n, int nx, int ny, int nz, const uint32_t *nbm,
+                                                          const double *tensor, int lstride, double dx, double dy, double dz,
+                                                          double alpha, unsigned long long *mismatch) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(c >= n) return;
+    const int z = (int)(c / ((int64_t)nx * ny));
+    double tz[3][6];
+    for(int l = 0; l < 3; l++) {
+        int zl = z - 1 + l;
+        zl = zl < 0 ? 0 : (zl >= nz ? nz - 1 : zl);
+        for(int k = 0; k < 6; k++) tz[l][k] = tensor[(int64_t)zl * lstride + k];
+    }
+    double row[27];
+    mask_aniso_row<double>(nbm[c], tz, dx, dy, dz, alpha, row);
+    bool bad = false;
+    for(int k = 0; k < 27; k++)
+        if(__double_as_longlong(row[k]) != __double_as_longlong(coef27[(int64_t)k * n + c])) bad = true;
+    if(bad) atomicAdd(mismatch, 1ull);
+}
+
+// Try mask mode for the FP32 view; returns 1 when every assembled row matched
+// its mask-formed row (op->nbm, lt32, geo32 set), 0 when not, -1 on error.
+static int try_mask_mode(const AssemblyParams &p, const double *coef27, Operator *op, hipStream_t s) {
+    const int64_t n = (int64_t)p.nx * p.ny * p.nz;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    uint32_t *nbm = nullptr;
+    unsigned long long *dmis = nullptr;
+    if(hipMalloc(&nbm, sizeof(uint32_t) * n) != hipSuccess || hipMalloc(&dmis, sizeof(unsigned long long)) != hipSuccess) {
+        if(nbm) (void)hipFree(nbm);
+        mono_set_error("out of device memory building the operator");
+        return -1;
+    }
+    const int lstride = p.tensor_source == kTensorPerLayer ? 6 : 0;
+    const int nl = lstride ? p.nz : 1;
+    std::vector<double> t64((size_t)nl * 6);
+    unsigned long long mismatches = 0;
+    k_nb_mask<<<grid, 256, 0, s>>>(p.nx, p.ny, p.nz, p.active, nbm);
+    (void)hipMemsetAsync(dmis, 0, sizeof(unsigned long long), s);
+    k_verify_mask_rows<<<grid, 256, 0, s>>>(coef27, n, p.nx, p.ny, p.nz, nbm, p.tensor, lstride, p.dx, p.dy, p.dz, p.alpha, dmis);
+    if(hipGetLastError() != hipSuccess ||
+       hipMemcpyAsync(&mismatches, dmis, sizeof(mismatches), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipMemcpyAsync(t64.data(), p.tensor, sizeof(double) * t64.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipFree(nbm);
+        (void)hipFree(dmis);
+        mono_set_error("HIP error verifying the operator's mask rows");
+        return -1;
+    }
+    (void)hipFree(dmis);
+    if(getenv("MONO_TRACE_MASK"))
+        fprintf(stderr, "[mono] mask rows: %llu of %lld rows differ from the assembled rows\n", mismatches, (long long)n);
+    if(mismatches != 0) {
+        (void)hipFree(nbm);
+        return 0;
+    }
+    // layer table [nz][6] in FP32 (a uniform tensor repeated)
+    std::vector<float> t32((size_t)p.nz * 6);
+    for(int z = 0; z < p.nz; z++)
+        for(int k = 0; k < 6; k++) t32[(size_t)z * 6 + k] = (float)t64[(size_t)(lstride ? z : 0) * 6 + k];
+    if(hipMalloc(&op->lt32, sizeof(float) * t32.size()) != hipSuccess ||
+       hipMemcpyAsync(op->lt32, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipFree(nbm);
+        mono_set_error("HIP error uploading the operator's layer tensors");
+        return -1;
+    }
+    op->nbm = nbm;
+    op->geo32[0] = (float)p.dx;
+    op->geo32[1] = (float)p.dy;
+    op->geo32[2] = (float)p.dz;
+    op->geo32[3] = (float)p.alpha;
+    return 1;
+}
+
 template <typename T>
 static OpView<T> make_view(const Operator &o, const T *table, const T *tint, const T *percell) {
     OpView<T> v;
@@ -544,16 +645,26 @@ OpView<float> Operator::view32() const {
         v.ctab = ctab32;
         return v;
     }
+    if(mask32) {
+        OpView<float> v = make_view<float>(*this, nullptr, nullptr, nullptr);
+        v.nbm = nbm;
+        v.lt = lt32;
+        for(int k = 0; k < 4; k++) v.geo[k] = geo32[k];
+        return v;
+    }
     return class_mode ? make_view<float>(*this, table32, tint32, nullptr) : make_view<float>(*this, nullptr, nullptr, percell32);
 }
 
 int Operator::release() {
-    void *bufs[] = {table64, tint64, table32, tint32, percell64, percell32, codes, ctab64, ctab32};
+    void *bufs[] = {table64, tint64, table32, tint32, percell64, percell32, codes, ctab64, ctab32, nbm, lt32};
     for(void *p : bufs)
         if(p) MONO_CHECK(hipFree(p));
     table64 = tint64 = percell64 = nullptr;
     table32 = tint32 = percell32 = nullptr;
     codes = nullptr;
+    nbm = nullptr;
+    lt32 = nullptr;
+    mask32 = false;
     ctab64 = nullptr;
     ctab32 = nullptr;
     code_mode = false;
@@ -699,11 +810,19 @@ int build_operator(const AssemblyParams &p, const double *coef27, Operator *op, 
                 return 0;
             }
         }
+        if(op->pat == 27 && p.aniso && !p.g && p.tensor_source != kTensorPerCell && p.nz > 1) {
+            const char *e = getenv("MONO_MASK_ROWS");
+            if(!e || strcmp(e, "0") != 0) {
+                const int rc = try_mask_mode(p, coef27, op, s);
+                if(rc < 0) return -1;
+                op->mask32 = rc == 1;
+            }
+        }
         int *dslots = nullptr;
         MONO_CHECK(hipMalloc(&dslots, sizeof(int) * op->pat));
         MONO_CHECK(hipMemcpyAsync(dslots, pslots.data(), sizeof(int) * op->pat, hipMemcpyHostToDevice, s));
         MONO_CHECK(hipMalloc(&op->percell64, sizeof(double) * op->pat * n));
-        MONO_CHECK(hipMalloc(&op->percell32, sizeof(float) * op->pat * n));
+        if(!op->mask32) MONO_CHECK(hipMalloc(&op->percell32, sizeof(float) * op->pat * n));
         k_compact<<<grid, 256, 0, s>>>(coef27, n, op->pat, dslots, op->percell64, op->percell32);
         MONO_CHECK(hipGetLastError());
         MONO_CHECK(hipStreamSynchronize(s));

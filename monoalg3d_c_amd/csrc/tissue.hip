@@ -288,7 +288,7 @@ int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out) {
         mono_set_error("hipFree failed");
         return fail();
     }
-    t->st.class_operator = t->op.class_mode ? 1 : (t->op.code_mode ? 2 : 0);
+    t->st.class_operator = t->op.class_mode ? 1 : (t->op.code_mode ? 2 : (t->op.mask32 ? 3 : 0));
     t->st.stencil_points = t->op.nslots;
 
     // state, potential, rhs
