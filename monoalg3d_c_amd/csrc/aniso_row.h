@@ -90,6 +90,7 @@ __host__ __device__ __forceinline__ void mask_flux_sigmas(uint32_t m, const T (&
 
 template <typename T> __host__ __device__ __forceinline__ T mask_qd(T s, int c) { return c != 0 ? s / T(c) : T(0); }
 
+// 1/x: the hardware reciprocal in FP32 (1 ulp), a division in FP64
 __host__ __device__ __forceinline__ float mask_rcp(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_rcpf(x);
@@ -97,6 +98,7 @@ __host__ __device__ __forceinline__ float mask_rcp(float x) {
     return 1.0f / x;
 #endif
 }
+__host__ __device__ __forceinline__ double mask_rcp(double x) { return 1.0 / x; }
 
 // Bits of the 27-point slots at offset o + t * e_b for the six pairs of the
 // flux through face (axis, sgn) along cross axis b (t = -1 gives the lower
@@ -124,19 +126,19 @@ __host__ __device__ constexpr PairMasks pair_masks(int axis, int sgn, int b) {
     return pm;
 }
 
-// FP32 flux sigmas of face (AXIS, SGN) with the two cross terms already
-// divided twice by their pair counts (the X values of mask_aniso_row): pair
+// Flux sigmas of face (AXIS, SGN) with the two cross terms already divided
+// twice by their pair counts (the X values of mask_aniso_row): pair
 // availability from the mask shifted by two cells along the cross axis, the
 // pairs of one layer counted at once, one reciprocal per cross term.  Same
-// rule as mask_flux_sigmas in another summation order: FP32 rounding apart.
-template <int AXIS, int SGN>
-__host__ __device__ __forceinline__ void mask_flux_fast(uint32_t m, const float (&tz)[3][6], float &s1, float &X2, float &X3) {
+// rule as mask_flux_sigmas in another summation order: rounding apart.
+template <int AXIS, int SGN, typename T>
+__host__ __device__ __forceinline__ void mask_flux_fast(uint32_t m, const T (&tz)[3][6], T &s1, T &X2, T &X3) {
     constexpr int oz = AXIS == 2 ? SGN : 0;
     constexpr int face = slot_of(AXIS == 0 ? SGN : 0, AXIS == 1 ? SGN : 0, oz);
     const bool has_face = (m >> face) & 1u;
-    s1 = has_face ? (tz[oz + 1][AXIS] + tz[1][AXIS]) * 0.5f : 0.0f;
+    s1 = has_face ? (tz[oz + 1][AXIS] + tz[1][AXIS]) * T(0.5) : T(0);
     constexpr int b0 = AXIS == 0 ? 1 : 0, b1 = AXIS == 2 ? 1 : 2;
-    float X[2];
+    T X[2];
 #pragma unroll
     for(int which = 0; which < 2; which++) {
         const int b = which == 0 ? b0 : b1;
@@ -145,19 +147,19 @@ __host__ __device__ __forceinline__ void mask_flux_fast(uint32_t m, const float 
         const int stride = b == 0 ? 1 : (b == 1 ? 3 : 9);
         const uint32_t avail = m & (m >> (2 * stride));  // bit q: cells q and q + 2 e_b both present
         const PairMasks pm = pair_masks(AXIS, SGN, b);
-        float sum = 0.0f;
+        T sum = T(0);
         int np = 0;
 #pragma unroll
         for(int g = 0; g < 3; g++) {
             if(!pm.g[g]) continue;
             const int c = __builtin_popcount(avail & pm.g[g]);
             // both cells of a pair: layers z-1 and z+1 when b is z, else layer z-1+g twice
-            const float K = b == 2 ? tz[2][cmp] + tz[0][cmp] : tz[g][cmp] + tz[g][cmp];
-            sum += (float)c * K;
+            const T K = b == 2 ? tz[2][cmp] + tz[0][cmp] : tz[g][cmp] + tz[g][cmp];
+            sum += T(c) * K;
             np += c;
         }
         // sum / cnt / cnt with cnt = 2 np
-        X[which] = (has_face && np) ? sum * mask_rcp((float)(4 * np * np)) : 0.0f;
+        X[which] = (has_face && np) ? sum * mask_rcp(T(4 * np * np)) : T(0);
     }
     X2 = X[0];
     X3 = X[1];
@@ -165,8 +167,8 @@ __host__ __device__ __forceinline__ void mask_flux_fast(uint32_t m, const float 
 
 // row[slot] (27-point slot order) of a cell with mask m; zero when the cell is
 // inactive (bit 13 clear).
-// FAST (FP32 only): the flux sigmas by mask_flux_fast; otherwise the
-// operations of k_assemble in its order.
+// FAST: the flux sigmas by mask_flux_fast; otherwise the operations of
+// k_assemble in its order.
 template <typename T, bool FAST = false>
 __host__ __device__ __forceinline__ void mask_aniso_row(uint32_t m, const T (&tz)[3][6], T dx, T dy, T dz, T alpha, T (&row)[27]) {
 #pragma unroll
@@ -177,7 +179,6 @@ __host__ __device__ __forceinline__ void mask_aniso_row(uint32_t m, const T (&tz
     // the twelve doubly divided cross terms
     T Xyr, Xzr, Xyl, Xzl, Yxt, Yzt, Yxd, Yzd, Zxf, Zyf, Zxb, Zyb;
     if constexpr(FAST) {
-        static_assert(sizeof(T) == 4, "the fast form is the FP32 one");
         mask_flux_fast<0, +1>(m, tz, x_r, Xyr, Xzr);
         mask_flux_fast<0, -1>(m, tz, x_l, Xyl, Xzl);
         mask_flux_fast<1, +1>(m, tz, y_t, Yxt, Yzt);

@@ -1107,14 +1107,14 @@ template <class F> static void with_pat(int P, F &&f) {
 
 // Call f(IC<S>, IC<PAT>) for the operator's class-row slot count and row pattern.
 // Call f(IC<CM>, IC<PAT>) for the operator's storage mode (0 per-cell rows,
-// 1 class table, 2 code table, 3 mask rows: FP32 views only) and row pattern.
+// 1 class table, 2 code table, 3 mask rows) and row pattern.
 template <typename TC, class F> static void with_op(const OpView<TC> &op, F &&f) {
     with_pat(op.pat, [&](auto P_) {
         if(op.table) f(IC<1>{}, P_);
         else if constexpr(decltype(P_)::value == 7) {
             if(op.codes) f(IC<2>{}, P_);
             else f(IC<0>{}, P_);
-        } else if constexpr(decltype(P_)::value == 27 && std::is_same<TC, float>::value) {
+        } else if constexpr(decltype(P_)::value == 27) {
             if(op.nbm) f(IC<3>{}, P_);
             else f(IC<0>{}, P_);
         } else f(IC<0>{}, P_);
@@ -1255,7 +1255,8 @@ int CgWorkspace::release() {
 
 template <typename T> static bool same_view(const OpView<T> &a, const OpView<T> &b) {
     return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.n == b.n && a.pat == b.pat && a.table == b.table &&
-           a.table_int == b.table_int && a.percell == b.percell && a.codes == b.codes && a.ctab == b.ctab && a.active == b.active;
+           a.table_int == b.table_int && a.percell == b.percell && a.codes == b.codes && a.ctab == b.ctab && a.active == b.active &&
+           a.nbm == b.nbm && a.lt == b.lt && a.geo[0] == b.geo[0] && a.geo[1] == b.geo[1] && a.geo[2] == b.geo[2] && a.geo[3] == b.geo[3];
 }
 
 static bool same_key(const CgWorkspace::PrefixKey &a, const CgWorkspace::PrefixKey &b) {

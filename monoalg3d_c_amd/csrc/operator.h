@@ -111,7 +111,9 @@ struct Operator {
     bool mask32 = false;   // FP32 view in mask mode (the FP64 view keeps per-cell rows)
     uint32_t *nbm = nullptr;
     float *lt32 = nullptr;
+    double *lt64 = nullptr;
     float geo32[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    double geo64[4] = {0.0, 0.0, 0.0, 0.0};
     uint8_t *codes = nullptr;
     double *ctab64 = nullptr;
     float *ctab32 = nullptr;
@@ -124,6 +126,10 @@ struct Operator {
 
     OpView<double> view64() const;
     OpView<float> view32() const;
+    // the FP64 view for the outer residuals of the mixed-precision solve: in
+    // mask mode, rows formed from the masks in FP64 (the assembled rows up to
+    // rounding) instead of the stored rows; else view64()
+    OpView<double> view64_outer() const;
     int release();
 };
 

@@ -594,12 +594,17 @@ static int try_mask_mode(const AssemblyParams &p, const double *coef27, Operator
         (void)hipFree(nbm);
         return 0;
     }
-    // layer table [nz][6] in FP32 (a uniform tensor repeated)
+    // layer tables [nz][6] in FP32 and FP64 (a uniform tensor repeated)
     std::vector<float> t32((size_t)p.nz * 6);
+    std::vector<double> l64((size_t)p.nz * 6);
     for(int z = 0; z < p.nz; z++)
-        for(int k = 0; k < 6; k++) t32[(size_t)z * 6 + k] = (float)t64[(size_t)(lstride ? z : 0) * 6 + k];
-    if(hipMalloc(&op->lt32, sizeof(float) * t32.size()) != hipSuccess ||
+        for(int k = 0; k < 6; k++) {
+            l64[(size_t)z * 6 + k] = t64[(size_t)(lstride ? z : 0) * 6 + k];
+            t32[(size_t)z * 6 + k] = (float)l64[(size_t)z * 6 + k];
+        }
+    if(hipMalloc(&op->lt32, sizeof(float) * t32.size()) != hipSuccess || hipMalloc(&op->lt64, sizeof(double) * l64.size()) != hipSuccess ||
        hipMemcpyAsync(op->lt32, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(op->lt64, l64.data(), sizeof(double) * l64.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
        hipStreamSynchronize(s) != hipSuccess) {
         (void)hipFree(nbm);
         mono_set_error("HIP error uploading the operator's layer tensors");
@@ -610,6 +615,10 @@ static int try_mask_mode(const AssemblyParams &p, const double *coef27, Operator
     op->geo32[1] = (float)p.dy;
     op->geo32[2] = (float)p.dz;
     op->geo32[3] = (float)p.alpha;
+    op->geo64[0] = p.dx;
+    op->geo64[1] = p.dy;
+    op->geo64[2] = p.dz;
+    op->geo64[3] = p.alpha;
     return 1;
 }
 
@@ -654,9 +663,17 @@ OpView<float> Operator::view32() const {
     }
     return class_mode ? make_view<float>(*this, table32, tint32, nullptr) : make_view<float>(*this, nullptr, nullptr, percell32);
 }
+OpView<double> Operator::view64_outer() const {
+    if(!mask32) return view64();
+    OpView<double> v = make_view<double>(*this, nullptr, nullptr, nullptr);
+    v.nbm = nbm;
+    v.lt = lt64;
+    for(int k = 0; k < 4; k++) v.geo[k] = geo64[k];
+    return v;
+}
 
 int Operator::release() {
-    void *bufs[] = {table64, tint64, table32, tint32, percell64, percell32, codes, ctab64, ctab32, nbm, lt32};
+    void *bufs[] = {table64, tint64, table32, tint32, percell64, percell32, codes, ctab64, ctab32, nbm, lt32, lt64};
     for(void *p : bufs)
         if(p) MONO_CHECK(hipFree(p));
     table64 = tint64 = percell64 = nullptr;
@@ -664,6 +681,7 @@ int Operator::release() {
     codes = nullptr;
     nbm = nullptr;
     lt32 = nullptr;
+    lt64 = nullptr;
     mask32 = false;
     ctab64 = nullptr;
     ctab32 = nullptr;

@@ -565,7 +565,7 @@ int mono_tissue_step(mono_tissue *t, int nsteps) {
         CgStats cs{};
         int rc;
         if(d.precision == MONO_FP64) rc = t->cg.solve_fp64(t->op.view64(), t->b, t->v, t->cgcfg, &cs, t->stream);
-        else rc = t->cg.solve_mixed(t->op.view64(), t->op.view32(), t->b, t->v, t->cgcfg, &cs, t->stream);
+        else rc = t->cg.solve_mixed(t->op.view64_outer(), t->op.view32(), t->b, t->v, t->cgcfg, &cs, t->stream);
         if(rc) return -1;
         if(t->timing) {
             MONO_CHECK(hipEventRecord(t->ev[2], t->stream));

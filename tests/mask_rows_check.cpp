@@ -32,7 +32,7 @@ int main() {
         for(int64_t i = 0; i < n; i++) act[i] = (rand() % 100) >= 10 * (trial % 3 + 1) ? 1 : 0;
         std::vector<double> coef(n * 27);
         orc_assemble_aniso(nx, ny, nz, dx, dy, dz, alpha, t6.data(), act.data(), coef.data());
-        int bad = 0, badf = 0;
+        int bad = 0, badf = 0, badd = 0;
         double maxrel = 0.0;
         for(int64_t c = 0; c < n; c++) {
             const int x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
@@ -52,6 +52,14 @@ int main() {
                 const double ref = coef[c * 27 + k];
                 if(memcmp(&ref, &row[k], 8) != 0) { if(bad < 3) printf("trial %d cell %ld slot %d: %.17g vs %.17g\n", trial, (long)c, k, row[k], ref); bad++; }
             }
+            // the fast form in FP64 (the mixed solve's outer residuals): within 1e-13 of the diagonal
+            double rowd[27];
+            mono::mask_aniso_row<double, true>(m, tz, dx, dy, dz, alpha, rowd);
+            for(int k = 0; k < 27; k++)
+                if(fabs(rowd[k] - coef[c * 27 + k]) > 1e-13 * fabs(coef[c * 27 + 13])) {
+                    if(badd < 3) printf("trial %d cell %ld slot %d: fast fp64 %.17g vs %.17g\n", trial, (long)c, k, rowd[k], coef[c * 27 + k]);
+                    badd++;
+                }
             // the FP32 form the inner solver uses: within 4e-6 of the row's diagonal
             float tzf[3][6], rowf[27];
             for(int l = 0; l < 3; l++) for(int k = 0; k < 6; k++) tzf[l][k] = (float)tz[l][k];
@@ -66,9 +74,9 @@ int main() {
                 maxrel = fmax(maxrel, scale > 0 ? err / scale : err);
             }
         }
-        printf("trial %d (%dx%dx%d): %d mismatching coefficients; fp32 form: %d outside 4e-6 of the diagonal, max %.2e\n", trial,
-               nx, ny, nz, bad, badf, maxrel);
-        bad_total += bad + badf;
+        printf("trial %d (%dx%dx%d): %d mismatching coefficients; fp32 form: %d outside 4e-6 of the diagonal, max %.2e; "
+               "fast fp64 form: %d outside 1e-13\n", trial, nx, ny, nz, bad, badf, maxrel, badd);
+        bad_total += bad + badf + badd;
     }
     return bad_total ? 1 : 0;
 }

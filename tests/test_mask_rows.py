@@ -6,9 +6,10 @@ instead of reading stored coefficients.  Built for the host, the same function
 in FP64 must reproduce the oracle's assembled rows (oracle/assembly.c,
 assembly_common.c:115-830) bit for bit; the operator builder repeats this
 check on the device against the assembled rows before it enables the mask.
-The FP32 form the kernels use (pair counts by popcount, one reciprocal per
-cross term) must agree with those rows to 4e-6 of each row's diagonal, and
-keep their zero pattern.
+The fast form the kernels use (pair counts by popcount, one reciprocal per
+cross term) must agree with those rows to 4e-6 of each row's diagonal in FP32
+(the inner solve), keeping their zero pattern, and to 1e-13 in FP64 (the
+outer residuals of the mixed-precision solve).
 """
 import os
 import shutil
@@ -34,3 +35,4 @@ def test_mask_rows_match_oracle_bitwise(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout
     assert r.stdout.count(" 0 mismatching coefficients; fp32 form: 0 outside") == 6
+    assert r.stdout.count("fast fp64 form: 0 outside") == 6
