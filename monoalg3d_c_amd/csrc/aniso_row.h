@@ -90,7 +90,9 @@ __host__ __device__ __forceinline__ void mask_flux_sigmas(uint32_t m, const T (&
 
 template <typename T> __host__ __device__ __forceinline__ T mask_qd(T s, int c) { return c != 0 ? s / T(c) : T(0); }
 
-// 1/x: the hardware reciprocal in FP32 (1 ulp), a division in FP64
+// 1/x: the hardware reciprocal in FP32 (1 ulp); in FP64 the hardware
+// reciprocal refined by two Newton steps (within an ulp or two; a division
+// costs three times the instructions)
 __host__ __device__ __forceinline__ float mask_rcp(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_rcpf(x);
@@ -98,7 +100,16 @@ __host__ __device__ __forceinline__ float mask_rcp(float x) {
     return 1.0f / x;
 #endif
 }
-__host__ __device__ __forceinline__ double mask_rcp(double x) { return 1.0 / x; }
+__host__ __device__ __forceinline__ double mask_rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+    r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+    return r;
+#else
+    return 1.0 / x;
+#endif
+}
 
 // Bits of the 27-point slots at offset o + t * e_b for the six pairs of the
 // flux through face (axis, sgn) along cross axis b (t = -1 gives the lower
@@ -158,8 +169,8 @@ __host__ __device__ __forceinline__ void mask_flux_fast(uint32_t m, const T (&tz
             sum += T(c) * K;
             np += c;
         }
-        // sum / cnt / cnt with cnt = 2 np
-        X[which] = (has_face && np) ? sum * mask_rcp(T(4 * np * np)) : T(0);
+        // sum / cnt / cnt with cnt = 2 np (sum is 0 when np is)
+        X[which] = has_face ? sum * mask_rcp(T(np ? 4 * np * np : 1)) : T(0);
     }
     X2 = X[0];
     X3 = X[1];
@@ -214,8 +225,10 @@ __host__ __device__ __forceinline__ void mask_aniso_row(uint32_t m, const T (&tz
 #pragma unroll
     for(int d = 0; d < kNDir; d++) acc[d] = T(0);
     auto has = [&](int d) { return (m >> dir_slot(d)) & 1u; };
+    // the exact form adds only non-zero terms, as k_assemble does (it decides
+    // the sign of a zero coefficient); the fast one adds every term
     auto upd = [&](int d, T v) {
-        if(v != T(0)) acc[d] += v;
+        if(FAST || v != T(0)) acc[d] += v;
     };
     const T diag = alpha + (yz_x * x_r + yz_x * x_l + xz_y * y_t + xz_y * y_d + xy_z * z_f + xy_z * z_b);
 #pragma unroll
@@ -421,7 +434,7 @@ __host__ __device__ __forceinline__ void mask_aniso_row(uint32_t m, const T (&tz
         default:
             break;
         }
-        if(v != T(0)) acc[d] += v;
+        if(FAST || v != T(0)) acc[d] += v;
     }
     row[13] = diag;
 #pragma unroll
