@@ -21,10 +21,10 @@ step of the atlas advances every instance by one step of its own run:
     instance is solved at least as tightly as its stand-alone run.  A relative
     test (cg_rel_tol, ||r|| <= rel ||b||) cannot be applied to the atlas as
     is: its ||b|| sums over all instances, so a small cable packed beside large
-    sheets would be solved far less tightly than alone.  The batch turns it
-    into the absolute test r.r <= rel^2 * min_i n_i * (alpha * 1 mV)^2, which
-    implies ||r_i|| <= rel ||b_i|| for every instance i whose RMS potential is
-    at least 1 mV (b_i = alpha V_i; a whole instance never sits at 0 mV).
+    sheets would be solved far less tightly than alone.  The library computes
+    each instance's ||b_i||^2 on the device after every ionic step and stops
+    on r.r <= rel^2 * min_i ||b_i||^2, which implies ||r_i|| <= rel ||b_i||
+    for every instance i (the reference's relative test, per instance).
 
 Instances share the model, dt, cell size, conductivity and the stimulus timing.
 A cable of n cells is the reference's initialize_grid_with_cable_mesh
@@ -172,11 +172,6 @@ class Batch:
         for bad in ("active", "stim_mask", "ionic_scale", "instance", "instance_scale", "g_cell"):
             if bad in kw:
                 raise ValueError(f"{bad} is set by the batch layout")
-        rel = kw.pop("cg_rel_tol", 0.0)
-        if rel and rel > 0.0:
-            from . import alpha as alpha_of
-            a = alpha_of(kw.get("beta", 0.14), kw.get("cm", 1.0), kw.get("dt", 0.02), *h)
-            kw["cg_tol"] = rel * rel * min(inst.n for inst in self.instances) * (a * 1.0) ** 2
         self.tissue = Tissue(self.layout.shape, h=h, model=model, active=self.layout.active(),
                              stim_mask=self.layout.stim_mask(h[0]), instance=self.layout.instance_ids(),
                              instance_scale=self.layout.scale_table(), **kw)

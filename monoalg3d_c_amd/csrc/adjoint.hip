@@ -362,6 +362,7 @@ extern "C" int mono_sensor_gradient(const mono_tissue_desc *desc, int nsteps, co
     acfg.mixed = false;
     acfg.jacobi = 0;
     acfg.tol = 0.0;
+    acfg.tol_dev = nullptr;
     // adjoint solves to ||r|| <= 1e-8 ||lambda|| (or the forward's relative
     // tolerance when that is looser): the gradient needs far less than the
     // forward's accuracy, and each digit costs about two CG iterations

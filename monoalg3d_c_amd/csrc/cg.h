@@ -43,6 +43,10 @@ struct CgConfig {
     double inner_rel = 1e-4;  // inner stop: ||r_in|| <= inner_rel * ||r_in0||
     int max_outer = 20;
     int poll_chunk = 0;       // iterations enqueued between status polls (0 = adaptive)
+    // if set: the outer r.r threshold, a device scalar written before the
+    // solve (replaces tol and rel_tol; batches of instances set it to
+    // rel^2 * min_i ||b_i||^2 each step)
+    const double *tol_dev = nullptr;
 };
 
 // Device buffers of a CSR solve: r, p, q, z (n doubles each), two partial-sum

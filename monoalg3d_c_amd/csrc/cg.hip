@@ -116,6 +116,7 @@ template <int NR> __device__ __forceinline__ void sum_partials(const double *__r
 struct CgSetup {
     double tol, rel_tol, inner_rel;
     int max_its, jacobi, max_outer, sweep;
+    const double *tol_dev;  // if set: the r.r threshold, read on the device (CgConfig::tol_dev)
 };
 
 // Start of an iteration kernel: settle the reduction of the previous kernel.
@@ -152,7 +153,7 @@ __device__ __forceinline__ bool iter_prologue(int mode, int k, const double *__r
     bool done;
     if(mode == 0) {
         const double rr = t[0], rz = cs.jacobi ? t[1] : t[0], bb = t[2];
-        const double tol_abs = cs.rel_tol > 0.0 ? cs.rel_tol * cs.rel_tol * bb : cs.tol;
+        const double tol_abs = cs.tol_dev ? *cs.tol_dev : (cs.rel_tol > 0.0 ? cs.rel_tol * cs.rel_tol * bb : cs.tol);
         done = !(rr >= tol_abs) || cs.max_its <= 1;
         beta = 0.0;
         if(blockIdx.x == 0 && threadIdx.x == 0) {
@@ -188,7 +189,8 @@ __device__ __forceinline__ bool iter_prologue(int mode, int k, const double *__r
         }
     } else {
         const double rr = t[0], bb = t[1];
-        const double tol_o = cs.sweep == 0 ? (cs.rel_tol > 0.0 ? cs.rel_tol * cs.rel_tol * bb : cs.tol) : so->tol_abs;
+        const double tol_o = cs.sweep == 0 ? (cs.tol_dev ? *cs.tol_dev : (cs.rel_tol > 0.0 ? cs.rel_tol * cs.rel_tol * bb : cs.tol))
+                                           : so->tol_abs;
         done = !(rr > tol_o) || cs.sweep >= cs.max_outer || (rr != rr);
         const double t_rel = cs.inner_rel * cs.inner_rel * rr;
         beta = 0.0;
@@ -1263,7 +1265,8 @@ static bool same_key(const CgWorkspace::PrefixKey &a, const CgWorkspace::PrefixK
     const CgConfig &u = a.cfg, &v = b.cfg;
     return a.kind == b.kind && a.sched == b.sched && same_view(a.op64, b.op64) && same_view(a.op32, b.op32) && a.b == b.b &&
            a.x == b.x && u.tol == v.tol && u.rel_tol == v.rel_tol && u.max_its == v.max_its && u.jacobi == v.jacobi &&
-           u.mixed == v.mixed && u.inner_rel == v.inner_rel && u.max_outer == v.max_outer && u.poll_chunk == v.poll_chunk;
+           u.mixed == v.mixed && u.inner_rel == v.inner_rel && u.max_outer == v.max_outer && u.poll_chunk == v.poll_chunk &&
+           u.tol_dev == v.tol_dev;
 }
 
 // First sight of a key: run eagerly and remember it.  Second sight: record the
@@ -1373,7 +1376,7 @@ static int launch_stencil(const OpView<TC> &op, const TV *x, const TV *rz, TV *p
 int CgWorkspace::fp64_launches(const OpView<double> &op, const double *b, double *x, const CgConfig &cfg, int k0, int count,
                                int &nres, hipStream_t s) {
     const int grid = grid_for(op.n);
-    const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, cfg.jacobi, cfg.max_outer, 0};
+    const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, cfg.jacobi, cfg.max_outer, 0, cfg.tol_dev};
     double *pb[2] = {p, p2};
     const double *rz = cfg.jacobi ? z : r;
     // boxes of more than one plane (the tiled stencil kernel): each
@@ -1503,7 +1506,7 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
     auto blind_sweeps = [&]() -> int {
         cur = 0;
         for(int sweep = 0; sweep < blind; sweep++) {
-            const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, 0, cfg.max_outer, sweep};
+            const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, 0, cfg.max_outer, sweep, cfg.tol_dev};
             const int nres = residual(sweep, cs);
             if(nres < 0) return -1;
             for(int k = 0; k < last_sched[sweep] + 2; k++)
@@ -1528,7 +1531,7 @@ int CgWorkspace::solve_mixed(const OpView<double> &op64, const OpView<float> &op
     bool blind_read = blind == 0;
     if(blind > 0) chunk0 = 1;  // the next sweep mostly confirms convergence
     for(int sweep = blind;; sweep++) {
-        const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, 0, cfg.max_outer, sweep};
+        const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, 0, cfg.max_outer, sweep, cfg.tol_dev};
         const int nres = residual(sweep, cs);
         if(nres < 0) return -1;
         MONO_CHECK(hipGetLastError());
@@ -1591,7 +1594,7 @@ int CgWorkspace::solve_csr(const CsrMatrix &A, const double *b, double *x, const
 int csr_solve(const CsrMatrix &A, const double *b, double *x, const CgConfig &cfg, CgStats *st, hipStream_t s, const CsrBuffers &w) {
     const int grid = grid_for(A.n);
     const unsigned sgrid = (unsigned)(((int64_t)A.n * 64 + kBlock - 1) / kBlock);
-    const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, cfg.jacobi, cfg.max_outer, 0};
+    const CgSetup cs{cfg.tol, cfg.rel_tol, cfg.inner_rel, cfg.max_its, cfg.jacobi, cfg.max_outer, 0, nullptr};
     double *r = w.r, *p = w.p, *q = w.q, *z = w.z, *P0 = w.P0, *P1 = w.P1;
     CgScalars *sc = w.sc, *host_sc = w.host_sc;
     k_csr_spmv<<<sgrid, kBlock, 0, s>>>(A.n, A.row_ptr, A.col, A.val, x, q, nullptr);

@@ -12,6 +12,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "../../include/monoalg_hip.h"
@@ -21,6 +23,9 @@
 #include "observer.h"
 #include "operator.h"
 #include "tissue_impl.h"
+
+// box rows (one (y, z) line each) per block of the per-instance ||b||^2 kernel
+static constexpr int kInstRows = 8;
 
 static thread_local char g_err[1024];
 
@@ -277,6 +282,43 @@ int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out) {
             mono_set_error("instance table upload failed");
             return fail();
         }
+        if(d.cg_rel_tol > 0.0) {
+            // the relative test per instance: ||r_i|| <= rel ||b_i|| for every
+            // instance i follows from r.r <= rel^2 * min_i ||b_i||^2
+            const int ni = d.n_instances;
+            std::vector<int> box((size_t)ni * 6);
+            for(int i = 0; i < ni; i++) {
+                box[(size_t)i * 6 + 0] = box[(size_t)i * 6 + 2] = box[(size_t)i * 6 + 4] = INT_MAX;
+                box[(size_t)i * 6 + 1] = box[(size_t)i * 6 + 3] = box[(size_t)i * 6 + 5] = -1;
+            }
+            for(int64_t c = 0; c < t->n; c++) {
+                if(d.active && !d.active[c]) continue;
+                int *bx = &box[(size_t)d.instance[c] * 6];
+                const int x = (int)(c % d.nx), y = (int)((c / d.nx) % d.ny), z = (int)(c / ((int64_t)d.nx * d.ny));
+                bx[0] = std::min(bx[0], x);
+                bx[1] = std::max(bx[1], x);
+                bx[2] = std::min(bx[2], y);
+                bx[3] = std::max(bx[3], y);
+                bx[4] = std::min(bx[4], z);
+                bx[5] = std::max(bx[5], z);
+            }
+            // chunks of kInstRows box rows per block: enough for the tallest box
+            int rows = 1;
+            for(int i = 0; i < ni; i++)
+                if(box[(size_t)i * 6 + 1] >= 0)
+                    rows = std::max(rows, (box[(size_t)i * 6 + 3] - box[(size_t)i * 6 + 2] + 1) *
+                                              (box[(size_t)i * 6 + 5] - box[(size_t)i * 6 + 4] + 1));
+            t->inst_nchunk = (rows + kInstRows - 1) / kInstRows;
+            if(hipMalloc(&t->inst_box, sizeof(int) * box.size()) != hipSuccess ||
+               hipMalloc(&t->inst_bb, sizeof(double) * ni * t->inst_nchunk) != hipSuccess ||
+               hipMalloc(&t->tol_dev, sizeof(double)) != hipSuccess ||
+               hipMemcpyAsync(t->inst_box, box.data(), sizeof(int) * box.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+               hipStreamSynchronize(s) != hipSuccess) {
+                mono_set_error("instance box upload failed");
+                return fail();
+            }
+            t->cgcfg.tol_dev = t->tol_dev;
+        }
     }
 
     // operator
@@ -347,7 +389,8 @@ int mono_tissue_destroy(mono_tissue *t) {
     if(t->op.release()) rc = -1;
     if(t->obs.release()) rc = -1;
     if(t->apd.release()) rc = -1;
-    void *bufs[] = {t->sv, t->v, t->b, t->active, t->g, t->tensor, t->stim_mask, t->ionic_scale, t->aux, t->inst, t->inst_scale};
+    void *bufs[] = {t->sv, t->v, t->b, t->active, t->g, t->tensor, t->stim_mask, t->ionic_scale, t->aux, t->inst, t->inst_scale,
+                    t->inst_box, t->inst_bb, t->tol_dev};
     for(void *p : bufs)
         if(p && hipFree(p) != hipSuccess) rc = -1;
     for(int k = 0; k < 3; k++)
@@ -503,6 +546,70 @@ static int launch_ionic(mono_tissue *t, double stim_amp, int block) {
 // trial's time is read back (its launch finished within the previous step's
 // solve, which waited on the device), then the next candidate, or the fastest
 // once every candidate has run twice after the first step.
+// Partial ||b_i||^2 of instance i (blockIdx.y) over rows kInstRows * blockIdx.x
+// .. + kInstRows - 1 of its bounding box (a row: one (y, z) line of the box),
+// in a fixed order: part[i * nchunk + chunk]
+__global__ __launch_bounds__(256) void k_instance_bb(const double *b, const uint16_t *inst, const uint8_t *active, const int *box,
+                                                     int nx, int ny, int nchunk, double *part) {
+    const int i = blockIdx.y, chunk = blockIdx.x;
+    const int *bx = box + 6 * i;
+    double s = 0.0;
+    if(bx[1] >= bx[0]) {
+        const int w = bx[1] - bx[0] + 1, h = bx[3] - bx[2] + 1, d = bx[5] - bx[4] + 1;
+        const int r0 = chunk * kInstRows, r1 = r0 + kInstRows < h * d ? r0 + kInstRows : h * d;
+        const int64_t cnt = r1 > r0 ? (int64_t)(r1 - r0) * w : 0;
+        for(int64_t k = threadIdx.x; k < cnt; k += 256) {
+            const int row = r0 + (int)(k / w);
+            const int x = bx[0] + (int)(k % w), y = bx[2] + row % h, z = bx[4] + row / h;
+            const int64_t c = x + (int64_t)nx * (y + (int64_t)ny * z);
+            if(inst[c] == i && (!active || active[c])) s += b[c] * b[c];
+        }
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for(int o = 128; o > 0; o >>= 1) {
+        if(threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if(threadIdx.x == 0) part[(int64_t)i * nchunk + chunk] = red[0];
+}
+
+// threshold rel^2 * min_i ||b_i||^2 over the instances that have cells: each
+// thread sums its instances' partials in chunk order, then a fixed-order min
+__global__ __launch_bounds__(256) void k_instance_tol(const double *part, const int *box, int ni, int nchunk, double rel2,
+                                                      double *tol) {
+    double m = -1.0;
+    for(int i = threadIdx.x; i < ni; i += 256) {
+        if(box[6 * i + 1] < box[6 * i]) continue;
+        double s = 0.0;
+        for(int c = 0; c < nchunk; c++) s += part[(int64_t)i * nchunk + c];
+        if(m < 0.0 || s < m) m = s;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for(int o = 128; o > 0; o >>= 1) {
+        if(threadIdx.x < o) {
+            const double a = red[threadIdx.x], b = red[threadIdx.x + o];
+            red[threadIdx.x] = a < 0.0 ? b : (b < 0.0 ? a : (b < a ? b : a));
+        }
+        __syncthreads();
+    }
+    if(threadIdx.x == 0) *tol = rel2 * (red[0] < 0.0 ? 0.0 : red[0]);
+}
+
+static int launch_instance_tolerance(mono_tissue *t) {
+    const mono_tissue_desc &d = t->d;
+    const int nchunk = t->inst_nchunk;
+    k_instance_bb<<<dim3(nchunk, d.n_instances), 256, 0, t->stream>>>(t->b, t->inst, t->active, t->inst_box, d.nx, d.ny, nchunk,
+                                                                         t->inst_bb);
+    k_instance_tol<<<1, 256, 0, t->stream>>>(t->inst_bb, t->inst_box, d.n_instances, nchunk, d.cg_rel_tol * d.cg_rel_tol,
+                                             t->tol_dev);
+    MONO_CHECK(hipGetLastError());
+    return 0;
+}
+
 static int tuned_block(mono_tissue *t, bool &timed) {
     auto &u = t->tune;
     timed = false;
@@ -562,6 +669,7 @@ int mono_tissue_step(mono_tissue *t, int nsteps) {
         if(launch_ionic(t, amp, block)) return -1;
         if(t->timing) MONO_CHECK(hipEventRecord(t->ev[1], t->stream));
         if(tuned) MONO_CHECK(hipEventRecord(t->tune.e1, t->stream));
+        if(t->tol_dev && launch_instance_tolerance(t)) return -1;
         CgStats cs{};
         int rc;
         if(d.precision == MONO_FP64) rc = t->cg.solve_fp64(t->op.view64(), t->b, t->v, t->cgcfg, &cs, t->stream);

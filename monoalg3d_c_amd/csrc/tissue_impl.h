@@ -24,6 +24,13 @@ struct mono_tissue {
     float *ionic_scale = nullptr;
     uint16_t *inst = nullptr;      // per-cell instance (batches)
     float *inst_scale = nullptr;   // n_instances x 4 factors
+    // batches with a relative CG tolerance: each instance's bounding box
+    // (x0, x1, y0, y1, z0, z1 of its active cells), its ||b||^2 and the
+    // solve's threshold rel^2 * min_i ||b_i||^2, set after every ionic step
+    int *inst_box = nullptr;
+    double *inst_bb = nullptr;  // partial sums, n_instances x inst_nchunk
+    int inst_nchunk = 0;
+    double *tol_dev = nullptr;
     double *aux = nullptr;  // adaptive ODE solver: dt, time_new, previous_dt rows of `pitch`
     mono::Operator op;
     mono::CgWorkspace cg;

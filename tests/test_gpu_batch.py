@@ -218,12 +218,15 @@ def test_sweep_matches_reference_runs(tmp_path):
 
 
 def test_batch_relative_tolerance_bounds_every_instance():
-    """cg_rel_tol on a batch: the atlas stops on the absolute bound
-    rel^2 * min_i n_i * (alpha * 1 mV)^2, so a 100-cell cable packed beside a
-    192x192 sheet is solved at least as tightly as alone with the relative
-    test: after every step of 10 ms the atlas residual r.r is below
-    rel^2 ||b_cable||^2 (b = alpha V*), and both instances match their
-    stand-alone runs with the relative test within VM_TOL."""
+    """cg_rel_tol on a batch: the atlas stops on r.r <= rel^2 min_i ||b_i||^2
+    (each instance's ||b_i||^2 taken on the device after every ionic step), so
+    a 100-cell cable packed beside a 192x192 sheet is solved at least as
+    tightly as alone with the relative test.  After every step of 10 ms the
+    atlas residual r.r is below rel^2 ||b_cable||^2; with b = A v and
+    ||A|| <= alpha + 4 * 0.013 on the cable (face coefficient sigma * area /
+    distance = 0.00013 * 100), ||b_cable||^2 <= 3.04 ||alpha v_cable||^2, the
+    bound checked here.  Both instances match their stand-alone runs with the
+    relative test within VM_TOL."""
     import monoalg3d_c_amd as m
     from monoalg3d_c_amd.batch import Batch, Instance
     announce("batch with cg_rel_tol: small cable beside a large sheet, 10 ms")
@@ -234,7 +237,7 @@ def test_batch_relative_tolerance_bounds_every_instance():
     for _ in range(500):
         b.step(1)
         cable = b.v()[0]
-        assert b.stats["last_cg_error"] <= rel * rel * float(((a * cable) ** 2).sum())
+        assert b.stats["last_cg_error"] <= rel * rel * 3.04 * float(((a * cable) ** 2).sum())
     got = b.v()
     b.close()
     for i, inst in enumerate(insts):
