@@ -24,8 +24,8 @@
 #include "operator.h"
 #include "tissue_impl.h"
 
-// box rows (one (y, z) line each) per block of the per-instance ||b||^2 kernel
-static constexpr int kInstRows = 8;
+// cells of a box row per wave of the per-instance ||b||^2 kernel
+static constexpr int kInstCols = 256;
 
 static thread_local char g_err[1024];
 
@@ -302,17 +302,33 @@ int mono_tissue_create(const mono_tissue_desc *desc, mono_tissue **out) {
                 bx[4] = std::min(bx[4], z);
                 bx[5] = std::max(bx[5], z);
             }
-            // chunks of kInstRows box rows per block: enough for the tallest box
-            int rows = 1;
-            for(int i = 0; i < ni; i++)
-                if(box[(size_t)i * 6 + 1] >= 0)
-                    rows = std::max(rows, (box[(size_t)i * 6 + 3] - box[(size_t)i * 6 + 2] + 1) *
-                                              (box[(size_t)i * 6 + 5] - box[(size_t)i * 6 + 4] + 1));
-            t->inst_nchunk = (rows + kInstRows - 1) / kInstRows;
+            // work items: (instance, box row, chunk of kInstCols columns), those
+            // of one instance consecutive from first[i]
+            std::vector<int4> items;
+            std::vector<int> first((size_t)ni + 1, 0);
+            for(int i = 0; i < ni; i++) {
+                first[i] = (int)items.size();
+                if(box[(size_t)i * 6 + 1] < 0) continue;
+                const int w = box[(size_t)i * 6 + 1] - box[(size_t)i * 6 + 0] + 1;
+                const int rows = (box[(size_t)i * 6 + 3] - box[(size_t)i * 6 + 2] + 1) * (box[(size_t)i * 6 + 5] - box[(size_t)i * 6 + 4] + 1);
+                for(int r = 0; r < rows; r++)
+                    for(int x = 0; x < w; x += kInstCols) items.push_back(make_int4(i, r, x, 0));
+            }
+            first[ni] = (int)items.size();
+            t->inst_nitems = (int)items.size();
+            if(t->inst_nitems < 1) {
+                mono_set_error("a batch with a relative CG tolerance has no active instance cells");
+                return fail();
+            }
             if(hipMalloc(&t->inst_box, sizeof(int) * box.size()) != hipSuccess ||
-               hipMalloc(&t->inst_bb, sizeof(double) * ni * t->inst_nchunk) != hipSuccess ||
+               hipMalloc(&t->inst_items, sizeof(int4) * items.size()) != hipSuccess ||
+               hipMalloc(&t->inst_first, sizeof(int) * first.size()) != hipSuccess ||
+               hipMalloc(&t->inst_part, sizeof(double) * items.size()) != hipSuccess ||
+               hipMalloc(&t->inst_bb, sizeof(double) * ni) != hipSuccess ||
                hipMalloc(&t->tol_dev, sizeof(double)) != hipSuccess ||
                hipMemcpyAsync(t->inst_box, box.data(), sizeof(int) * box.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+               hipMemcpyAsync(t->inst_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+               hipMemcpyAsync(t->inst_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
                hipStreamSynchronize(s) != hipSuccess) {
                 mono_set_error("instance box upload failed");
                 return fail();
@@ -390,7 +406,7 @@ int mono_tissue_destroy(mono_tissue *t) {
     if(t->obs.release()) rc = -1;
     if(t->apd.release()) rc = -1;
     void *bufs[] = {t->sv, t->v, t->b, t->active, t->g, t->tensor, t->stim_mask, t->ionic_scale, t->aux, t->inst, t->inst_scale,
-                    t->inst_box, t->inst_bb, t->tol_dev};
+                    t->inst_box, t->inst_items, t->inst_first, t->inst_part, t->inst_bb, t->tol_dev};
     for(void *p : bufs)
         if(p && hipFree(p) != hipSuccess) rc = -1;
     for(int k = 0; k < 3; k++)
@@ -546,45 +562,48 @@ static int launch_ionic(mono_tissue *t, double stim_amp, int block) {
 // trial's time is read back (its launch finished within the previous step's
 // solve, which waited on the device), then the next candidate, or the fastest
 // once every candidate has run twice after the first step.
-// Partial ||b_i||^2 of instance i (blockIdx.y) over rows kInstRows * blockIdx.x
-// .. + kInstRows - 1 of its bounding box (a row: one (y, z) line of the box),
-// in a fixed order: part[i * nchunk + chunk]
-__global__ __launch_bounds__(256) void k_instance_bb(const double *b, const uint16_t *inst, const uint8_t *active, const int *box,
-                                                     int nx, int ny, int nchunk, double *part) {
-    const int i = blockIdx.y, chunk = blockIdx.x;
+// Partial ||b_i||^2 of one work item (one wave): up to kInstCols cells of one
+// row of instance i's bounding box (a row: one (y, z) line of the box), all
+// loads issued at once, summed in a fixed order into part[item]
+__global__ __launch_bounds__(64) void k_instance_part(const double *b, const uint16_t *inst, const uint8_t *active, const int *box,
+                                                      const int4 *items, int nx, int ny, double *part) {
+    const int4 it = items[blockIdx.x];  // instance, box row, first column of the chunk
+    const int i = it.x;
     const int *bx = box + 6 * i;
+    const int w = bx[1] - bx[0] + 1, h = bx[3] - bx[2] + 1;
+    const int y = bx[2] + it.y % h, z = bx[4] + it.y / h;
+    const int64_t c0 = bx[0] + (int64_t)nx * (y + (int64_t)ny * z);
+    double v[kInstCols / 64];
+#pragma unroll
+    for(int u = 0; u < kInstCols / 64; u++) {
+        const int x = it.z + u * 64 + (int)threadIdx.x;
+        const int64_t c = c0 + (x < w ? x : 0);
+        const bool in = x < w && inst[c] == i && (!active || active[c]);
+        v[u] = in ? b[c] : 0.0;
+    }
     double s = 0.0;
-    if(bx[1] >= bx[0]) {
-        const int w = bx[1] - bx[0] + 1, h = bx[3] - bx[2] + 1, d = bx[5] - bx[4] + 1;
-        const int r0 = chunk * kInstRows, r1 = r0 + kInstRows < h * d ? r0 + kInstRows : h * d;
-        const int64_t cnt = r1 > r0 ? (int64_t)(r1 - r0) * w : 0;
-        for(int64_t k = threadIdx.x; k < cnt; k += 256) {
-            const int row = r0 + (int)(k / w);
-            const int x = bx[0] + (int)(k % w), y = bx[2] + row % h, z = bx[4] + row / h;
-            const int64_t c = x + (int64_t)nx * (y + (int64_t)ny * z);
-            if(inst[c] == i && (!active || active[c])) s += b[c] * b[c];
-        }
-    }
-    __shared__ double red[256];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for(int o = 128; o > 0; o >>= 1) {
-        if(threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if(threadIdx.x == 0) part[(int64_t)i * nchunk + chunk] = red[0];
+#pragma unroll
+    for(int u = 0; u < kInstCols / 64; u++) s += v[u] * v[u];
+    for(int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+    if(threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// threshold rel^2 * min_i ||b_i||^2 over the instances that have cells: each
-// thread sums its instances' partials in chunk order, then a fixed-order min
-__global__ __launch_bounds__(256) void k_instance_tol(const double *part, const int *box, int ni, int nchunk, double rel2,
-                                                      double *tol) {
+// ||b_i||^2 of instance i (one wave): its items' partials, lane-strided, then
+// a fixed-order wave reduction; -1 for an instance without cells
+__global__ __launch_bounds__(64) void k_instance_sum(const double *part, const int *first, double *bb) {
+    const int i = blockIdx.x;
+    double s = 0.0;
+    for(int k = first[i] + (int)threadIdx.x; k < first[i + 1]; k += 64) s += part[k];
+    for(int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+    if(threadIdx.x == 0) bb[i] = first[i + 1] > first[i] ? s : -1.0;
+}
+
+// threshold rel^2 * min_i ||b_i||^2 over the instances that have cells
+__global__ __launch_bounds__(256) void k_instance_tol(const double *bb, int ni, double rel2, double *tol) {
     double m = -1.0;
     for(int i = threadIdx.x; i < ni; i += 256) {
-        if(box[6 * i + 1] < box[6 * i]) continue;
-        double s = 0.0;
-        for(int c = 0; c < nchunk; c++) s += part[(int64_t)i * nchunk + c];
-        if(m < 0.0 || s < m) m = s;
+        const double s = bb[i];
+        if(s >= 0.0 && (m < 0.0 || s < m)) m = s;
     }
     __shared__ double red[256];
     red[threadIdx.x] = m;
@@ -601,11 +620,10 @@ __global__ __launch_bounds__(256) void k_instance_tol(const double *part, const 
 
 static int launch_instance_tolerance(mono_tissue *t) {
     const mono_tissue_desc &d = t->d;
-    const int nchunk = t->inst_nchunk;
-    k_instance_bb<<<dim3(nchunk, d.n_instances), 256, 0, t->stream>>>(t->b, t->inst, t->active, t->inst_box, d.nx, d.ny, nchunk,
-                                                                         t->inst_bb);
-    k_instance_tol<<<1, 256, 0, t->stream>>>(t->inst_bb, t->inst_box, d.n_instances, nchunk, d.cg_rel_tol * d.cg_rel_tol,
-                                             t->tol_dev);
+    k_instance_part<<<t->inst_nitems, 64, 0, t->stream>>>(t->b, t->inst, t->active, t->inst_box, t->inst_items, d.nx, d.ny,
+                                                           t->inst_part);
+    k_instance_sum<<<d.n_instances, 64, 0, t->stream>>>(t->inst_part, t->inst_first, t->inst_bb);
+    k_instance_tol<<<1, 256, 0, t->stream>>>(t->inst_bb, d.n_instances, d.cg_rel_tol * d.cg_rel_tol, t->tol_dev);
     MONO_CHECK(hipGetLastError());
     return 0;
 }

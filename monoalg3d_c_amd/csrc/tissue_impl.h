@@ -28,8 +28,11 @@ struct mono_tissue {
     // (x0, x1, y0, y1, z0, z1 of its active cells), its ||b||^2 and the
     // solve's threshold rel^2 * min_i ||b_i||^2, set after every ionic step
     int *inst_box = nullptr;
-    double *inst_bb = nullptr;  // partial sums, n_instances x inst_nchunk
-    int inst_nchunk = 0;
+    int4 *inst_items = nullptr;  // (instance, box row, first column) per partial sum
+    int *inst_first = nullptr;   // first item of each instance, n_instances + 1
+    double *inst_part = nullptr; // partial sums, one per item
+    double *inst_bb = nullptr;   // ||b_i||^2 per instance
+    int inst_nitems = 0;
     double *tol_dev = nullptr;
     double *aux = nullptr;  // adaptive ODE solver: dt, time_new, previous_dt rows of `pitch`
     mono::Operator op;
