@@ -34,8 +34,10 @@ import argparse
 import json
 import os
 import re
+import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -267,8 +269,7 @@ def cpu_baseline(max_threads=16, shape=(256, 256, 32), steps=12, skip=2):
     if not os.path.exists(REF_EXE):
         return None
     threads = max(1, min(os.cpu_count() or 1, max_threads))
-    out_dir = os.path.join(ROOT, "gpurun_out", "cpu_baseline")
-    os.makedirs(out_dir, exist_ok=True)
+    out_dir = tempfile.mkdtemp(prefix="mono_cpu_baseline_")
     mesh, fib = os.path.join(out_dir, "mesh.alg"), os.path.join(out_dir, "fibres.txt")
     t0 = time.time()
     n = write_custom_mesh(mesh, shape, 100.0)
@@ -283,8 +284,7 @@ def cpu_baseline(max_threads=16, shape=(256, 256, 32), steps=12, skip=2):
         fh.write(_ref_ini(threads, DT * (steps - 1), domain, asm, "shared_libs/libohara_rudy_endo_2011.so", tol))
     r = subprocess.run([REF_EXE, "-c", ini], cwd=REF_DIR, capture_output=True, text=True, timeout=900)
     wall = time.time() - t0
-    for p in (mesh, fib):
-        os.remove(p)
+    shutil.rmtree(out_dir, ignore_errors=True)
     if r.returncode != 0:
         return None
     its = _iteration_times(r.stdout)
@@ -314,8 +314,7 @@ def dropin_rates(T=10.0):
     n = int(np.prod(shape))
     out = {"workload": "config 1: 64x64x16 TT2006, isotropic 0.13 S/m, CG r.r <= 1e-16, dt 0.02 ms", "simulated_ms": T}
     if os.path.exists(REF_EXE) and os.path.exists(mono) and os.path.exists(ode):
-        out_dir = os.path.join(ROOT, "gpurun_out", "dropin")
-        os.makedirs(out_dir, exist_ok=True)
+        out_dir = tempfile.mkdtemp(prefix="mono_dropin_")
         domain = ("name=Cuboid Mesh\nstart_dx=100.0\nstart_dy=100.0\nstart_dz=100.0\nside_length_x=6400.0\n"
                   "side_length_y=6400.0\nside_length_z=1600.0\nmain_function=initialize_grid_with_cuboid_mesh")
         asm = f"library_file={mono}\nsigma_x=0.00013\nsigma_y=0.00013\nsigma_z=0.00013\nmain_function=mono_homogeneous_sigma_assembly_matrix"
@@ -327,6 +326,7 @@ def dropin_rates(T=10.0):
         with open(path, "w") as fh:
             fh.write(ini)
         r = subprocess.run([REF_EXE, "-c", path], cwd=REF_DIR, capture_output=True, text=True, timeout=600)
+        shutil.rmtree(out_dir, ignore_errors=True)
         its = _iteration_times(r.stdout) if r.returncode == 0 else []
         if len(its) > 10:
             out["reference_core_with_plugins"] = dict(rate=n / float(np.mean(its[10:])), ms_per_step=1e3 * float(np.mean(its[10:])),
