@@ -178,8 +178,113 @@ __host__ __device__ __forceinline__ void mask_flux_fast(uint32_t m, const T (&tz
 
 // row[slot] (27-point slot order) of a cell with mask m; zero when the cell is
 // inactive (bit 13 clear).
-// FAST: the flux sigmas by mask_flux_fast; otherwise the operations of
-// k_assemble in its order.
+// The 26 off-diagonal coefficients of the anisotropic rule (the direction
+// cases of mask_aniso_row) with neighbour presence as factors h[d] in {0, 1}:
+// a case contributes only when its own neighbour is present (factor g), an
+// inner term only when its partner is (factor h of the partner).
+template <typename T>
+__host__ __device__ __forceinline__ void mask_assemble_fast(const T (&h)[kNDir], T dx, T dy, T dz, T yz_x, T xz_y, T xy_z, T x_r,
+                                                            T x_l, T y_t, T y_d, T z_f, T z_b, T Xyr, T Xzr, T Xyl, T Xzl, T Yxt,
+                                                            T Yzt, T Yxd, T Yzd, T Zxf, T Zyf, T Zxb, T Zyb, T (&row)[27]) {
+    T acc[kNDir];
+#pragma unroll
+    for(int d = 0; d < kNDir; d++) acc[d] = T(0);
+    T g, s1, s2, s3;
+    // faces
+    acc[kFR] += h[kFR] * (-z_f * xy_z + h[kBK] * (-Yzt * dx + Yzd * dx - Xzr * dy + Xzl * dy));
+    acc[kBK] += h[kBK] * (-z_b * xy_z + h[kFR] * (Yzt * dx - Yzd * dx + Xzr * dy - Xzl * dy));
+    acc[kTP] += h[kTP] * (-y_t * xz_y + h[kDN] * (-Zyf * dx + Zyb * dx - Xyr * dz + Xyl * dz));
+    acc[kDN] += h[kDN] * (-y_d * xz_y + h[kTP] * (Zyf * dx - Zyb * dx + Xyr * dz - Xyl * dz));
+    acc[kRT] += h[kRT] * (-x_r * yz_x + h[kLF] * (-Zxf * dy + Zxb * dy - Yxt * dz + Yxd * dz));
+    acc[kLF] += h[kLF] * (-x_l * yz_x + h[kRT] * (Zxf * dy - Zxb * dy + Yxt * dz - Yxd * dz));
+    // edges
+    g = h[kFR_TP];
+    s1 = h[kFR_DN] * (-Xyr * dz + Xyl * dz - Zyf * dx);
+    s2 = h[kBK_TP] * (-Yzt * dx - Xzr * dy + Xzl * dy);
+    acc[kFR_DN] -= g * s1;
+    acc[kBK_TP] -= g * s2;
+    acc[kFR_TP] += g * (s1 + s2);
+    g = h[kFR_DN];
+    s1 = h[kBK_DN] * (Yzd * dx - Xzr * dy + Xzl * dy);
+    acc[kFR_DN] += g * s1;
+    acc[kBK_DN] -= g * s1;
+    g = h[kBK_TP];
+    s1 = h[kBK_DN] * (Zyb * dx - Xyr * dz + Xyl * dz);
+    acc[kBK_TP] += g * s1;
+    acc[kBK_DN] -= g * s1;
+    g = h[kFR_RT];
+    s1 = h[kBK_RT] * (-Xzr * dy - Yzt * dx + Yzd * dx);
+    s2 = h[kFR_LF] * (-Zxf * dy - Yxt * dz + Yxd * dz);
+    acc[kBK_RT] -= g * s1;
+    acc[kFR_LF] -= g * s2;
+    acc[kFR_RT] += g * (s1 + s2);
+    g = h[kFR_LF];
+    s1 = h[kBK_LF] * (-Yzt * dx + Yzd * dx + Xzl * dy);
+    acc[kFR_LF] += g * s1;
+    acc[kBK_LF] -= g * s1;
+    g = h[kBK_RT];
+    s1 = h[kBK_LF] * (Zxb * dy - Yxt * dz + Yxd * dz);
+    acc[kBK_RT] += g * s1;
+    acc[kBK_LF] -= g * s1;
+    g = h[kTP_RT];
+    s1 = h[kDN_RT] * (-Xyr * dz - Zyf * dx + Zyb * dx);
+    s2 = h[kTP_LF] * (-Zxf * dy + Zxb * dy - Yxt * dz);
+    acc[kDN_RT] -= g * s1;
+    acc[kTP_LF] -= g * s2;
+    acc[kTP_RT] += g * (s1 + s2);
+    g = h[kTP_LF];
+    s1 = h[kDN_LF] * (-Zyf * dx + Zyb * dx + Xyl * dz);
+    acc[kTP_LF] += g * s1;
+    acc[kDN_LF] -= g * s1;
+    g = h[kDN_RT];
+    s1 = h[kDN_LF] * (-Zxf * dy + Zxb * dy + Yxd * dz);
+    acc[kDN_RT] += g * s1;
+    acc[kDN_LF] -= g * s1;
+    // corners
+    g = h[kFR_TP_RT];
+    s1 = h[kFR_DN_RT] * (-Xyr * dz - Zyf * dx);
+    s2 = h[kBK_TP_RT] * (-Xzr * dy - Yzt * dx);
+    s3 = h[kFR_TP_LF] * (-Zxf * dy - Yxt * dz);
+    acc[kFR_DN_RT] -= g * s1;
+    acc[kBK_TP_RT] -= g * s2;
+    acc[kFR_TP_LF] -= g * s3;
+    acc[kFR_TP_RT] += g * (s1 + s2 + s3);
+    g = h[kFR_TP_LF];
+    s1 = h[kFR_DN_LF] * (Xyl * dz - Zyf * dx);
+    s2 = h[kBK_TP_LF] * (Xzl * dy - Yzt * dx);
+    acc[kFR_TP_LF] += g * (s1 + s2);
+    acc[kFR_DN_LF] -= g * s1;
+    acc[kBK_TP_LF] -= g * s2;
+    g = h[kFR_DN_RT];
+    s1 = h[kBK_DN_RT] * (-Xzr * dy + Yzd * dx);
+    s2 = h[kFR_DN_LF] * (Yxd * dz - Zxf * dy);
+    acc[kFR_DN_RT] += g * (s1 + s2);
+    acc[kBK_DN_RT] -= g * s1;
+    acc[kFR_DN_LF] -= g * s2;
+    g = h[kFR_DN_LF];
+    s1 = h[kBK_DN_LF] * (Xzl * dy + Yzd * dx);
+    acc[kFR_DN_LF] += g * s1;
+    acc[kBK_DN_LF] -= g * s1;
+    g = h[kBK_TP_RT];
+    s1 = h[kBK_DN_RT] * (-Xyr * dz + Zyb * dx);
+    s2 = h[kBK_TP_LF] * (-Yxt * dz + Zxb * dy);
+    acc[kBK_TP_RT] += g * (s1 + s2);
+    acc[kBK_DN_RT] -= g * s1;
+    acc[kBK_TP_LF] -= g * s2;
+    g = h[kBK_TP_LF];
+    s1 = h[kBK_DN_LF] * (Xyl * dz + Zyb * dx);
+    acc[kBK_TP_LF] += g * s1;
+    acc[kBK_DN_LF] -= g * s1;
+    g = h[kBK_DN_RT];
+    s1 = h[kBK_DN_LF] * (Yxd * dz + Zxb * dy);
+    acc[kBK_DN_RT] += g * s1;
+    acc[kBK_DN_LF] -= g * s1;
+#pragma unroll
+    for(int d = 0; d < kNDir; d++) row[dir_slot(d)] = acc[d];
+}
+
+// FAST: the flux sigmas by mask_flux_fast and the coefficients by
+// mask_assemble_fast; otherwise the operations of k_assemble in its order.
 template <typename T, bool FAST = false>
 __host__ __device__ __forceinline__ void mask_aniso_row(uint32_t m, const T (&tz)[3][6], T dx, T dy, T dz, T alpha, T (&row)[27]) {
 #pragma unroll
@@ -221,6 +326,17 @@ __host__ __device__ __forceinline__ void mask_aniso_row(uint32_t m, const T (&tz
         Zyb = mask_qd(yz_jz_b, cyz_jz_b);
     }
 
+    if constexpr(FAST) {
+        // every neighbour test a factor h[d] of 0 or 1: straight-line
+        // multiply-adds in place of compares and selects
+        T h[kNDir];
+#pragma unroll
+        for(int d = 0; d < kNDir; d++) h[d] = T((m >> dir_slot(d)) & 1u);
+        mask_assemble_fast(h, dx, dy, dz, yz_x, xz_y, xy_z, x_r, x_l, y_t, y_d, z_f, z_b, Xyr, Xzr, Xyl, Xzl, Yxt, Yzt, Yxd, Yzd,
+                           Zxf, Zyf, Zxb, Zyb, row);
+        row[13] = alpha + (yz_x * x_r + yz_x * x_l + xz_y * y_t + xz_y * y_d + xy_z * z_f + xy_z * z_b);
+        return;
+    }
     T acc[kNDir];
 #pragma unroll
     for(int d = 0; d < kNDir; d++) acc[d] = T(0);
