@@ -63,7 +63,7 @@ def workload_desc(shape, precision, active=None):
     return d
 
 
-def bytes_per_cell_step(neq, w_state, k_inner, mixed, outer, coef_slots=0):
+def bytes_per_cell_step(neq, w_state, k_inner, mixed, outer, coef_slots=0, mask=False):
     """Algorithmic HBM bytes per active cell-step (compulsory traffic of the
     algorithm, perfect reuse of stencil neighbours).
 
@@ -75,12 +75,14 @@ def bytes_per_cell_step(neq, w_state, k_inner, mixed, outer, coef_slots=0):
     first, the FP64 correction (d FP32 in, x FP64 in/out: 20 B); the inner
     iterations 11 FP32 words each.  Operators stored per cell add coef_slots
     coefficients (FP64 in FP64 sweeps, FP32 in the inner ones) per stencil
-    application; per-layer class tables add none."""
+    application; per-layer class tables add none; rows formed from
+    neighbourhood masks add the 4-byte mask per stencil application."""
     ionic = 2 * (neq - 1) * w_state + 8 + 8
+    mb = 4 if mask else 0
     if not mixed:
-        return ionic + 4 * 8 + k_inner * 11 * 8 + (1 + k_inner) * coef_slots * 8
-    return (ionic + outer * (8 + 8 + 4 + coef_slots * 8) + max(outer - 1, 0) * (4 + 8 + 8)
-            + k_inner * (11 * 4 + coef_slots * 4))
+        return ionic + 4 * 8 + k_inner * 11 * 8 + (1 + k_inner) * (coef_slots * 8 + mb)
+    return (ionic + outer * (8 + 8 + 4 + coef_slots * 8 + mb) + max(outer - 1, 0) * (4 + 8 + 8)
+            + k_inner * (11 * 4 + coef_slots * 4 + mb))
 
 
 def run_precision(shape, precision, steps, warmup, active=None):
@@ -115,10 +117,10 @@ def run_precision(shape, precision, steps, warmup, active=None):
                 stencil_points=s1["stencil_points"])
 
 
-def phase_roofline(r, neq, mixed, coef_slots):
+def phase_roofline(r, neq, mixed, coef_slots, mask=False):
     """Per-phase algorithmic bandwidth of a run_precision result (GB/s)."""
     w = 4 if mixed else 8
-    step_b = bytes_per_cell_step(neq, w, r["inner_its"], mixed, r["outer"], coef_slots)
+    step_b = bytes_per_cell_step(neq, w, r["inner_its"], mixed, r["outer"], coef_slots, mask)
     ionic_b = (2 * (neq - 1) * w + 16) * r["cells"]
     solve_b = (step_b - (2 * (neq - 1) * w + 16)) * r["cells"]
     return dict(ionic=dict(achieved=ionic_b / (r["ionic_ms"] * 1e-3) / 1e9, bytes=ionic_b, ms=r["ionic_ms"]),
@@ -480,7 +482,8 @@ def make_line(args, shape, neq, res, res3, errors, cpu, dropin) -> dict:
     head = res.get("fp32") or res["fp64"]
     mixed = "fp32" in res
     rf = {k: phase_roofline(r, neq, k == "fp32", 0) for k, r in res.items()}
-    rf3 = {k: phase_roofline(r, neq, k == "fp32", 19 if r["class_operator"] == 0 else 0) for k, r in res3.items()}
+    rf3 = {k: phase_roofline(r, neq, k == "fp32", 19 if r["class_operator"] == 0 else 0, r["class_operator"] == 3)
+           for k, r in res3.items()}
 
     # The dominant kernel is the ionic step: one launch per step, more device
     # time than any one kernel of the solve.  Its launch time is the ionic

@@ -663,7 +663,7 @@ OpView<float> Operator::view32() const {
     }
     return class_mode ? make_view<float>(*this, table32, tint32, nullptr) : make_view<float>(*this, nullptr, nullptr, percell32);
 }
-OpView<double> Operator::view64_outer() const {
+OpView<double> Operator::view64_solve() const {
     if(!mask32) return view64();
     OpView<double> v = make_view<double>(*this, nullptr, nullptr, nullptr);
     v.nbm = nbm;

@@ -2,13 +2,13 @@
 
 On a fibrotic box with layered anisotropic fibres (the 16M-cell workload's
 operator at 48 x 40 x 12) the operator builder enables mask mode after the
-FP64 mask rows reproduced every assembled row bit for bit; the mixed solve
-then forms its FP32 inner rows and FP64 outer residual rows from the masks.
-Against the same run with stored rows (MONO_MASK_ROWS=0 when the tissue is
-created): the same CG iteration counts and Vm within 1e-4 mV over 200 steps
-(the two differ by the rounding of the formed rows, ~1e-7 of a row's diagonal
-in FP32, ~1e-16 in FP64).  The FP64 solve keeps the stored rows in either
-case: bit for bit the same states.
+FP64 mask rows reproduced every assembled row bit for bit; the solves then
+form their rows from the masks (the FP64 CG and the mixed solve's outer
+residuals in FP64, its inner sweeps in FP32).  Against the same run with
+stored rows (MONO_MASK_ROWS=0 when the tissue is created): the same CG
+iteration counts and Vm within 1e-4 mV over 200 steps for FP32+refine, within
+1e-7 mV over 100 steps for FP64 (the two differ by the rounding of the formed
+rows, ~1e-7 of a row's diagonal in FP32, ~1e-16 in FP64).
 """
 import numpy as np
 import pytest
@@ -51,11 +51,16 @@ def test_mask_rows_match_stored_rows_fp32_refine(monkeypatch):
     assert (v1 > -30.0).any() and (v1 < -70.0).any()
 
 
-def test_fp64_solve_keeps_stored_rows(monkeypatch):
+def test_fp64_mask_rows_match_stored_rows(monkeypatch):
     import monoalg3d_c_amd as m
-    announce(f"FP64 solve with and without mask mode, fibrotic anisotropic {SHAPE}")
+    announce(f"mask rows vs stored rows, FP64, fibrotic anisotropic {SHAPE}")
     v1, s1, st1 = _run(m, monkeypatch, m.FP64, True, nsteps=100)
     v0, s0, st0 = _run(m, monkeypatch, m.FP64, False, nsteps=100)
-    np.testing.assert_array_equal(v1, v0)
-    np.testing.assert_array_equal(s1, s0)
+    assert st1["class_operator"] == 3 and st0["class_operator"] == 0
     assert st1["cg_iterations_total"] == st0["cg_iterations_total"]
+    d = float(np.abs(v1 - v0).max())
+    ds = float(np.abs(s1 - s0).max())
+    print(f"[mask rows] FP64 max |dV| mask vs stored rows over 100 steps: {d:.3e} mV, "
+          f"max |d state| {ds:.3e}", flush=True)
+    assert d < 1e-7, d
+    assert (v1 > -30.0).any() and (v1 < -70.0).any()

@@ -99,3 +99,24 @@ def test_aniso_tissue_matches_oracle(oracle):
     # stored slot union is 27
     assert st["class_operator"] == 1 and st["stencil_points"] == 27
     assert worst < 1e-6, worst
+
+
+def test_aniso_fibrotic_tissue_matches_oracle(oracle):
+    """Layered fibres with 10% inactive cells: the rows differ cell by cell, so
+    the FP64 solve forms them from 4-byte neighbourhood masks (mask mode,
+    class_operator 3); the oracle multiplies by its assembled rows.  Vm within
+    1e-6 mV, activation times within one step."""
+    shape = (16, 14, 6)
+    n = 16 * 14 * 6
+    rng = np.random.default_rng(33)
+    act = (rng.random(n) >= 0.1).astype(np.uint8)
+    act[:16 * 14] = 1  # keep the stimulated face connected
+    announce(f"tissue aniso fibrotic {shape} tt2006 6 ms (mask rows)")
+    worst, lr, lg, st, _ = _run_pair(oracle, shape, "tt2006", 300, aniso=True, active=act)
+    assert st["class_operator"] == 3 and st["stencil_points"] == 27
+    assert worst < 1e-6, worst
+    live = act.astype(bool)
+    fin = np.isfinite(lr) & live
+    assert fin.sum() > 50
+    assert np.array_equal(fin, np.isfinite(lg) & live)
+    assert np.abs(lr[fin] - lg[fin]).max() <= 0.02 + 1e-12
