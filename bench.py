@@ -519,7 +519,8 @@ def make_line(args, shape, neq, res, res3, errors, cpu, dropin) -> dict:
         "phase_roofline": rf,
         "fibrotic": None if not res3 else {
             "workload": "the same cuboid with 10%% of the cells non-conducting (SplitMix64 seed %d, left out as inactive "
-                        "cells; rows stored per cell)" % FIBROSIS_SEED,
+                        "cells; each row formed from the cell's 4-byte neighbourhood mask and its layer's tensor when "
+                        "class_operator is 3, else stored per cell)" % FIBROSIS_SEED,
             "active_cells": res3[next(iter(res3))]["cells"], "fp64": res3.get("fp64"), "fp32_refine": res3.get("fp32"),
             "speedup_fp32_refine_vs_fp64": (res3["fp32"]["rate"] / res3["fp64"]["rate"]) if len(res3) == 2 else None,
             "phase_roofline": rf3},
