@@ -126,10 +126,11 @@ struct Operator {
 
     OpView<double> view64() const;
     OpView<float> view32() const;
-    // the FP64 view the tissue solves read (the FP64 CG, and the outer
-    // residuals of the mixed-precision solve): in mask mode, rows formed from
-    // the masks in FP64 (the assembled rows up to rounding) instead of the
-    // stored rows; else view64()
+    // the FP64 view the tissue solves read (the FP64 CG without Jacobi, and
+    // the outer residuals of the mixed-precision solve): in mask mode, rows
+    // formed from the masks in FP64 (the assembled rows up to rounding)
+    // instead of the stored rows; else view64().  Its percell is null, so a
+    // Jacobi solve (diag_of reads percell) takes view64()
     OpView<double> view64_solve() const;
     int release();
 };

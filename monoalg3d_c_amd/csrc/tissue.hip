@@ -690,7 +690,10 @@ int mono_tissue_step(mono_tissue *t, int nsteps) {
         if(t->tol_dev && launch_instance_tolerance(t)) return -1;
         CgStats cs{};
         int rc;
-        if(d.precision == MONO_FP64) rc = t->cg.solve_fp64(t->op.view64_solve(), t->b, t->v, t->cgcfg, &cs, t->stream);
+        // the Jacobi update reads each row's diagonal from the stored rows, so a
+        // preconditioned FP64 solve keeps them (mask-mode views store none)
+        if(d.precision == MONO_FP64)
+            rc = t->cg.solve_fp64(t->cgcfg.jacobi ? t->op.view64() : t->op.view64_solve(), t->b, t->v, t->cgcfg, &cs, t->stream);
         else rc = t->cg.solve_mixed(t->op.view64_solve(), t->op.view32(), t->b, t->v, t->cgcfg, &cs, t->stream);
         if(rc) return -1;
         if(t->timing) {
