@@ -137,8 +137,9 @@ typedef struct mono_stats {
     int64_t outer_sweeps_total;
     int nan_detected;               /* the reference stops on a NaN solver error */
     int class_operator;             /* operator storage: 0 per-cell rows, 1 per-layer class table,
-                                       2 neighbour-code table (one byte per cell), 3 per-cell FP64
-                                       rows with FP32 rows formed from a 4-byte neighbour mask */
+                                       2 neighbour-code table (one byte per cell), 3 rows formed
+                                       in the solves from a 4-byte neighbour mask per cell (FP32
+                                       and FP64; stored FP64 rows kept for Jacobi FP64 solves) */
     int stencil_points;             /* stored non-zeros per row */
     double ionic_ms_total;          /* device time in the ionic kernel (timing on) */
     double solve_ms_total;          /* device time in the diffusion solve (timing on) */
