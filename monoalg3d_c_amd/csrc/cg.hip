@@ -785,8 +785,20 @@ __global__ __launch_bounds__(kBlock) void k_stencil2d(OpView<TC> op, int nitems,
 }
 
 // Residual kernels: no prologue.  Iteration kernel (q = A p): prologue first.
+// Waves per SIMD asked of the class-mode FP32 sweeps: at 6 the compiler keeps
+// them at 73 VGPRs without spills (unbounded they take 81, 5 waves), and the
+// occupancy query gives the work split 6 blocks per CU.  Measured on the 16M
+// box: FP32+refine step 2.36 ms at 6, 2.40 at 7, 2.42 unbounded; the FP64
+// class sweep bounded to 4 waves (124 VGPRs) was slower.  Other storage
+// modes spill under such bounds and keep the compiler's choice (1: none).
+#ifndef MONO_SW32
+#define MONO_SW32 6
+#endif
+template <typename TV, int CM> struct SweepWaves {
+    static constexpr int value = (CM == 1 && sizeof(TV) == 4) ? MONO_SW32 : 1;
+};
 template <typename TV, typename TC, int CM, int PAT, bool FUSE, class Epi>
-__global__ __launch_bounds__(kBlock) void k_stencil(OpView<TC> op, TileGeom g, const TV *__restrict__ x, const TV *__restrict__ rz,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<TV, CM>::value))) void k_stencil(OpView<TC> op, TileGeom g, const TV *__restrict__ x, const TV *__restrict__ rz,
                                                     TV *__restrict__ p_out, Epi epi, CgSetup cs, int mode, int k, const double *Pin,
                                                     int nprev, double *Pout, CgScalars *sc, CgScalars *so) {
     double betad = 0.0;
